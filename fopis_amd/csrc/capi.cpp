@@ -1,0 +1,410 @@
+// extern "C" boundary of libfopis_hip.so (include/fopis.h): argument checks,
+// status codes, the context lifecycle and the order of the step's stages.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "fopis_internal.hpp"
+
+namespace fopis {
+
+static thread_local std::string g_last_error;
+
+void timer_begin(Ctx& c, const char* name, hipEvent_t* ev) {
+  if (!c.timing) return;
+  FOPIS_HIP(hipEventCreate(ev));
+  FOPIS_HIP(hipEventRecord(*ev, c.stream));
+  (void)name;
+}
+
+void timer_end(Ctx& c, const char* name, hipEvent_t ev) {
+  if (!c.timing || !ev) return;
+  hipEvent_t e2;
+  if (hipEventCreate(&e2) != hipSuccess) return;
+  if (hipEventRecord(e2, c.stream) != hipSuccess) return;
+  PhaseTimer& t = c.timers[name];
+  t.pending.emplace_back(ev, e2);
+  if (t.pending.size() > 512) timers_collect(c);
+}
+
+void timers_collect(Ctx& c) {
+  for (auto& kv : c.timers) {
+    PhaseTimer& t = kv.second;
+    for (auto& pr : t.pending) {
+      if (hipEventSynchronize(pr.second) == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+          t.total_ms += ms;
+          t.n += 1;
+        }
+      }
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    t.pending.clear();
+  }
+}
+
+static void check_desc(const fopis_desc& d) {
+  if (d.nx < 4 || d.ny < 4 || d.nz < 4) throw Error(FOPIS_E_ARG, "nx, ny, nz must be >= 4");
+  if (d.nspecies < 1 || d.nspecies > FOPIS_MAX_SPECIES) throw Error(FOPIS_E_ARG, "nspecies out of range");
+  if (std::strlen(d.boundary) != 6) throw Error(FOPIS_E_ARG, "boundary must have 6 letters");
+  for (int q = 0; q < 6; ++q) {
+    const char b = d.boundary[q];
+    if (b == 'P') throw Error(FOPIS_E_ARG, "periodic faces are not supported by this build");
+    if (b != 'D' && b != 'N')
+      throw Error(FOPIS_E_ARG, std::string("Unknown boundary condition type was used: boundary = ") + b);
+  }
+  for (int s = 0; s < d.nspecies; ++s)
+    if (!(d.species[s].mass > 0.0)) throw Error(FOPIS_E_ARG, "species mass must be positive");
+  if (!(d.eps0 > 0.0)) throw Error(FOPIS_E_ARG, "eps0 must be positive");
+  if (d.precision != FOPIS_FP64 && d.precision != FOPIS_FP32) throw Error(FOPIS_E_ARG, "bad precision");
+  if (d.mg_precision != FOPIS_FP64 && d.mg_precision != FOPIS_FP32) throw Error(FOPIS_E_ARG, "bad mg precision");
+}
+
+static void update_rho(Ctx& c, bool push) {
+  particles_deposit_bin(c, push);
+  objects_sum_whole_charge(c);
+  if (c.nobj > 0) {
+    double rr = 0.0;
+    poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+    objects_redistribute(c);
+  }
+}
+
+}  // namespace fopis
+
+using namespace fopis;
+
+struct fopis_ctx {
+  Ctx c;
+};
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return FOPIS_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return FOPIS_E_STATE;
+  }
+}
+
+#define NEED_CTX                                 \
+  if (!ctx) {                                    \
+    g_last_error = "null context";               \
+    return FOPIS_E_ARG;                          \
+  }
+
+extern "C" {
+
+const char* fopis_last_error(void) { return g_last_error.c_str(); }
+
+int fopis_device_info(char* buf, int buflen) {
+  return guarded([&] {
+    int dev = 0;
+    FOPIS_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t pr;
+    FOPIS_HIP(hipGetDeviceProperties(&pr, dev));
+    std::string s = std::string("libfopis_hip 0.1; device ") + pr.name + " (" + pr.gcnArchName + "), " +
+                    std::to_string(pr.multiProcessorCount) + " CUs, " +
+                    std::to_string((long long)(pr.totalGlobalMem >> 20)) + " MiB";
+    if (buf && buflen > 0) {
+      std::strncpy(buf, s.c_str(), buflen - 1);
+      buf[buflen - 1] = 0;
+    }
+  });
+}
+
+int fopis_create(const fopis_desc* desc, fopis_ctx** out) {
+  if (!desc || !out) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  *out = nullptr;
+  fopis_ctx* h = new fopis_ctx();
+  const int rc = guarded([&] {
+    Ctx& c = h->c;
+    c.d = *desc;
+    check_desc(c.d);
+    if (c.d.mg_omega == 0.0) c.d.mg_omega = 1.5;
+    if (c.d.mg_pre == 0) c.d.mg_pre = 2;
+    if (c.d.mg_post == 0) c.d.mg_post = 2;
+    if (c.d.mg_max_cycles == 0) c.d.mg_max_cycles = 100;
+    if (c.d.poisson_tol == 0.0) c.d.poisson_tol = 1e-8;
+    if (c.d.capacity_slack == 0.0) c.d.capacity_slack = 1.25;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(FOPIS_E_DEVICE, "no HIP device");
+    c.S = c.d.nspecies;
+    c.nx = c.d.nx;
+    c.ny = c.d.ny;
+    c.nz = c.d.nz;
+    c.cx = c.nx + 2;
+    c.cy = c.ny + 2;
+    c.cz = c.nz + 2;
+    c.Nc[0] = c.nx - 1;
+    c.Nc[1] = c.ny - 1;
+    c.Nc[2] = c.nz - 1;
+    for (int q = 0; q < 3; ++q) c.nt[q] = div_up(c.Nc[q], kTile);
+    c.ntiles = c.nt[0] * c.nt[1] * c.nt[2];
+    c.nnode = (size_t)c.cx * c.cy * c.cz;
+    c.ncell_d = (size_t)(c.cx - 1) * (c.cy - 1) * (c.cz - 1);
+    c.fp32 = c.d.precision == FOPIS_FP32;
+    FOPIS_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    fields_init(c);
+    particles_init(c);
+    poisson_init(c);
+  });
+  if (rc != FOPIS_OK) {
+    if (h->c.stream) (void)hipStreamDestroy(h->c.stream);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return FOPIS_OK;
+}
+
+int fopis_destroy(fopis_ctx* ctx) {
+  if (!ctx) return FOPIS_OK;
+  (void)hipStreamSynchronize(ctx->c.stream);
+  timers_collect(ctx->c);
+  hipStream_t st = ctx->c.stream;
+  delete ctx;
+  if (st) (void)hipStreamDestroy(st);
+  return FOPIS_OK;
+}
+
+int fopis_set_particles(fopis_ctx* ctx, int s, int64_t n, const double* x, const double* y, const double* z,
+                        const double* vx, const double* vy, const double* vz) {
+  NEED_CTX
+  return guarded([&] {
+    if (s < 0 || s >= ctx->c.S) throw Error(FOPIS_E_ARG, "species out of range");
+    if (n < 0 || (n > 0 && (!x || !y || !z || !vx || !vy || !vz))) throw Error(FOPIS_E_ARG, "bad particle arrays");
+    const double* h[6] = {x, y, z, vx, vy, vz};
+    particles_set(ctx->c, s, n, h);
+  });
+}
+
+int fopis_num_particles(fopis_ctx* ctx, int s, int64_t* n) {
+  NEED_CTX
+  return guarded([&] {
+    if (s < 0 || s >= ctx->c.S || !n) throw Error(FOPIS_E_ARG, "bad argument");
+    *n = particles_count(ctx->c, s);
+  });
+}
+
+int fopis_get_particles(fopis_ctx* ctx, int s, int64_t cap, double* x, double* y, double* z, double* vx, double* vy,
+                        double* vz, int64_t* n) {
+  NEED_CTX
+  return guarded([&] {
+    if (s < 0 || s >= ctx->c.S || !n) throw Error(FOPIS_E_ARG, "bad argument");
+    double* h[6] = {x, y, z, vx, vy, vz};
+    particles_get(ctx->c, s, cap, h, n);
+  });
+}
+
+int fopis_add_particles(fopis_ctx* ctx, int s, int64_t n, const double* x, const double* y, const double* z,
+                        const double* vx, const double* vy, const double* vz) {
+  NEED_CTX
+  return guarded([&] {
+    if (s < 0 || s >= ctx->c.S) throw Error(FOPIS_E_ARG, "species out of range");
+    if (n < 0 || (n > 0 && (!x || !y || !z || !vx || !vy || !vz))) throw Error(FOPIS_E_ARG, "bad particle arrays");
+    const double* h[6] = {x, y, z, vx, vy, vz};
+    particles_stage(ctx->c, s, n, h);
+  });
+}
+
+int fopis_sort_by_cell(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] { particles_sort_by_cell(ctx->c); });
+}
+
+int fopis_get_cell_keys(fopis_ctx* ctx, int s, int64_t cap, int32_t* keys, int64_t* n) {
+  NEED_CTX
+  return guarded([&] {
+    if (s < 0 || s >= ctx->c.S || !n) throw Error(FOPIS_E_ARG, "bad argument");
+    particles_cell_keys(ctx->c, s, cap, keys, n);
+  });
+}
+
+int fopis_update_particle_velocity_es(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] { particles_velocity(ctx->c); });
+}
+
+int fopis_update_particle_position_es(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] { particles_position(ctx->c); });
+}
+
+int fopis_update_density(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] { particles_density_only(ctx->c); });
+}
+
+int fopis_update_rho(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] { update_rho(ctx->c, false); });
+}
+
+int fopis_solve_poisson(fopis_ctx* ctx, int32_t* cycles, double* relres) {
+  NEED_CTX
+  int rc = guarded([&] {
+    Ctx& c = ctx->c;
+    double rr = 0.0;
+    const int cyc = poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+    if (cycles) *cycles = cyc;
+    if (relres) *relres = rr;
+    if (!(rr <= c.d.poisson_tol)) throw Error(FOPIS_E_NOCONV, "Poisson solve stopped at relres " + std::to_string(rr));
+  });
+  return rc;
+}
+
+int fopis_update_efield(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] { fields_efield(ctx->c); });
+}
+
+int fopis_initialize(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] {
+    Ctx& c = ctx->c;
+    update_rho(c, false);
+    double rr = 0.0;
+    poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+    fields_efield(c);
+  });
+}
+
+int fopis_main_loop_es(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] {
+    Ctx& c = ctx->c;
+    update_rho(c, true);
+    double rr = 0.0;
+    poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+    if (!(rr <= c.d.poisson_tol)) throw Error(FOPIS_E_NOCONV, "Poisson solve stopped at relres " + std::to_string(rr));
+    fields_efield(c);
+    ++c.timestep;
+  });
+}
+
+int fopis_get_field(fopis_ctx* ctx, int which, int s, double* out) {
+  NEED_CTX
+  return guarded([&] {
+    if (!out) throw Error(FOPIS_E_ARG, "null output");
+    fields_get(ctx->c, which, s, out);
+  });
+}
+
+int fopis_set_field(fopis_ctx* ctx, int which, int s, const double* in) {
+  NEED_CTX
+  return guarded([&] {
+    if (!in) throw Error(FOPIS_E_ARG, "null input");
+    fields_set(ctx->c, which, s, in);
+  });
+}
+
+int fopis_energy(fopis_ctx* ctx, double* out) {
+  NEED_CTX
+  return guarded([&] {
+    if (!out) throw Error(FOPIS_E_ARG, "null output");
+    out[0] = particles_energy(ctx->c);
+    out[1] = fields_energy(ctx->c);
+  });
+}
+
+int fopis_set_object(fopis_ctx* ctx, int obj, const uint8_t* cell_mask, int32_t ncmat, const int32_t* cmat_ijk,
+                     int32_t fixed, double fixed_potential) {
+  NEED_CTX
+  return guarded([&] {
+    if (!cell_mask || (ncmat > 0 && !cmat_ijk)) throw Error(FOPIS_E_ARG, "null object arrays");
+    objects_set(ctx->c, obj, cell_mask, ncmat, cmat_ijk, fixed, fixed_potential);
+  });
+}
+
+int fopis_build_cmatrix(fopis_ctx* ctx, int obj) {
+  NEED_CTX
+  return guarded([&] {
+    if (obj < 0 || obj >= ctx->c.nobj) throw Error(FOPIS_E_ARG, "object index out of range");
+    objects_build_cmatrix(ctx->c, obj);
+  });
+}
+
+int fopis_get_cmatrix(fopis_ctx* ctx, int obj, double* out) {
+  NEED_CTX
+  return guarded([&] {
+    if (obj < 0 || obj >= ctx->c.nobj || !out) throw Error(FOPIS_E_ARG, "bad argument");
+    objects_get_cmatrix(ctx->c, obj, out);
+  });
+}
+
+int fopis_set_cmatrix(fopis_ctx* ctx, int obj, const double* in) {
+  NEED_CTX
+  return guarded([&] {
+    if (obj < 0 || obj >= ctx->c.nobj || !in) throw Error(FOPIS_E_ARG, "bad argument");
+    objects_set_cmatrix(ctx->c, obj, in);
+  });
+}
+
+int fopis_add_object_charge(fopis_ctx* ctx, int obj, const double* dq, const double* dcurrent) {
+  NEED_CTX
+  return guarded([&] {
+    if (obj < 0 || obj >= ctx->c.nobj) throw Error(FOPIS_E_ARG, "object index out of range");
+    objects_add_charge(ctx->c, obj, dq, dcurrent);
+  });
+}
+
+int fopis_get_object_state(fopis_ctx* ctx, int obj, double* potential, double* total_charge, double* current,
+                           double* charge_map) {
+  NEED_CTX
+  return guarded([&] {
+    if (obj < 0 || obj >= ctx->c.nobj) throw Error(FOPIS_E_ARG, "object index out of range");
+    objects_state(ctx->c, obj, potential, total_charge, current, charge_map);
+  });
+}
+
+int fopis_set_object_charge_map(fopis_ctx* ctx, int obj, const double* charge_map) {
+  NEED_CTX
+  return guarded([&] {
+    if (obj < 0 || obj >= ctx->c.nobj || !charge_map) throw Error(FOPIS_E_ARG, "bad argument");
+    objects_set_charge_map(ctx->c, obj, charge_map);
+  });
+}
+
+int fopis_phase_time(fopis_ctx* ctx, const char* phase, double* avg_ms, int64_t* n) {
+  NEED_CTX
+  return guarded([&] {
+    if (!phase || !avg_ms || !n) throw Error(FOPIS_E_ARG, "null argument");
+    timers_collect(ctx->c);
+    auto it = ctx->c.timers.find(phase);
+    if (it == ctx->c.timers.end() || it->second.n == 0) {
+      *avg_ms = 0.0;
+      *n = 0;
+      return;
+    }
+    *avg_ms = it->second.total_ms / (double)it->second.n;
+    *n = it->second.n;
+  });
+}
+
+int fopis_reset_timers(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] {
+    timers_collect(ctx->c);
+    ctx->c.timers.clear();
+  });
+}
+
+int fopis_synchronize(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] { FOPIS_HIP(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+// Internal types of libfopis_hip.so: device context, particle bins, launch
+// helpers. Not part of the ABI (include/fopis.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fopis.h"
+
+namespace fopis {
+
+// Edge of a particle tile in cells. Particles are binned per 8^3-cell tile;
+// one workgroup owns a tile per launch and keeps the tile's E nodes and its
+// charge window in LDS.
+constexpr int kTile = 8;
+constexpr int kEW = kTile + 1;  // E-field window: nodes of the tile's cells
+constexpr int kRW = kTile + 3;  // charge window: tile cells +-1 after a move
+constexpr int kDW = kTile + 2;  // density window: cells of the tile +-1
+constexpr int kBlock = 256;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define FOPIS_HIP(call)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      throw ::fopis::Error(FOPIS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+// Device buffer owned by the context.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) return;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      throw Error(FOPIS_E_NOMEM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    n = count;
+  }
+  void swap(DBuf& o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
+};
+
+// Six SoA component arrays of one particle set.
+template <class R>
+struct Soa {
+  DBuf<R> c[6];
+  void alloc(size_t n) {
+    for (auto& a : c) a.alloc(n);
+  }
+  void release() {
+    for (auto& a : c) a.release();
+  }
+  size_t size() const { return c[0].n; }
+};
+
+// Raw pointers of a SoA set, passed to kernels by value.
+template <class R>
+struct SoaPtr {
+  R* c[6];
+};
+
+// Device bookkeeping of one species.
+template <class R>
+struct SpeciesBins {
+  Soa<R> reg;            // tile regions
+  DBuf<int64_t> start;   // [ntiles] region start
+  DBuf<int32_t> capa;    // [ntiles] region capacity
+  DBuf<int32_t> cnt;     // [ntiles] live particles in the region
+  Soa<R> out;            // per-tile outboxes of particles leaving the tile
+  DBuf<uint8_t> odir;    // direction (0..26) of each outbox entry
+  DBuf<int64_t> ostart;  // [ntiles]
+  DBuf<int32_t> ocapa;   // [ntiles]
+  DBuf<int32_t> ocnt;    // [ntiles]
+  Soa<double> inbox;     // staged new particles (FP64 from the host), sorted by tile
+  DBuf<int32_t> ibeg;    // [ntiles+1]
+  int64_t inbox_n = 0;
+  std::vector<double> host_inbox[6];  // staged on the host until the next deposit
+  Soa<R> stray;          // particles that did not fit a bin this step
+  int64_t stray_cap = 0;
+  int64_t total_cap = 0;
+};
+
+struct ObjectDev {
+  bool defined = false;
+  int ncmat = 0;
+  bool fixed = false;
+  double fixed_potential = 0;
+  DBuf<int32_t> node_cmat;   // [(nx+2)^3] glue node -> cmat index or -1
+  DBuf<int64_t> cmat_node;   // [ncmat] glue linear node index
+  DBuf<double> C;            // [ncmat^2], element (col,row) at col*n+row
+  DBuf<double> colsum;       // [ncmat] sum_i C(i,j)
+  double total_cmat = 0;
+  DBuf<double> tmp_charge;   // [S][ncmat] temporary_charge_map
+  DBuf<double> charge_map;   // [S][ncmat]
+  DBuf<unsigned long long> absorbed;  // [S] absorbed superparticles this step
+  DBuf<double> scal;         // [4]: potential, total_charge, ctp, scratch
+  std::vector<double> current;         // [S] host copy (Spacecraft::current)
+  std::vector<double> host_current_add;  // emission currents added by the host
+  double potential = 0, total_charge = 0;
+  std::vector<int32_t> cmat_ijk;       // host copy
+};
+
+// Per-phase event timer.
+struct PhaseTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double total_ms = 0;
+  int64_t n = 0;
+};
+
+struct MgLevel {
+  int n[3];            // interior unknowns per axis
+  int64_t sx, sy;      // strides (elements) of the level array
+  int64_t off = 0;     // element offset of the level's (0,0,0) node in its arrays
+  size_t size;         // elements per level array
+  DBuf<double> e64, r64, t64;  // FP64 level arrays (used when the V-cycle is FP64)
+  DBuf<float> e32, r32, t32;   // FP32 level arrays
+};
+
+struct Ctx {
+  fopis_desc d{};
+  int S = 0;
+  int nx = 0, ny = 0, nz = 0;  // node counts (RootGrid nx)
+  int cx = 0, cy = 0, cz = 0;  // with glue
+  int Nc[3] = {0, 0, 0};       // cells per axis = n - 1
+  int nt[3] = {0, 0, 0};       // tiles per axis
+  int ntiles = 0;
+  size_t nnode = 0;            // cx*cy*cz
+  size_t ncell_d = 0;          // (cx-1)^3 density cells
+  bool fp32 = false;
+  hipStream_t stream = nullptr;
+  std::string devname;
+  int xcds = 8;
+
+  // fields (FP64, reference glue layout)
+  DBuf<double> phi, rho, density, eref;   // rho: (S+1)*nnode, density: S*ncell_d, eref: 3*nnode
+  DBuf<double> rho_part;                  // [ntiles][kRW^3] per species launch
+  DBuf<int32_t> dens_part;                // [ntiles][kDW^3]
+  DBuf<double> red;                       // reduction scratch
+  DBuf<int32_t> status;                   // [8] stray counts etc.
+  DBuf<uint8_t> mask;                     // (nx+1)^3 conductor cell ids (0 none, o+1)
+  bool have_mask = false;
+
+  std::vector<SpeciesBins<double>> sp64;
+  std::vector<SpeciesBins<float>> sp32;
+
+  ObjectDev obj[FOPIS_MAX_OBJECTS];
+  int nobj = 0;
+
+  std::vector<MgLevel> mg;
+  int mg_cycles_last = 0;
+  double relres_last = 0;
+
+  std::map<std::string, PhaseTimer> timers;
+  bool timing = true;
+  int timestep = 1;
+
+  size_t id3(int i, int j, int k) const { return ((size_t)i * cy + j) * cz + k; }
+};
+
+// ---- entry points implemented in the .hip units ----
+void particles_init(Ctx& c);
+void particles_set(Ctx& c, int s, int64_t n, const double* const* host6);
+int64_t particles_count(Ctx& c, int s);
+void particles_get(Ctx& c, int s, int64_t cap, double* const* host6, int64_t* n);
+void particles_stage(Ctx& c, int s, int64_t n, const double* const* host6);
+void particles_velocity(Ctx& c);
+void particles_position(Ctx& c);
+void particles_density_only(Ctx& c);
+void particles_deposit_bin(Ctx& c, bool push);
+void particles_sort_by_cell(Ctx& c);
+void particles_cell_keys(Ctx& c, int s, int64_t cap, int32_t* keys, int64_t* n);
+double particles_energy(Ctx& c);
+
+void fields_init(Ctx& c);
+void fields_efield(Ctx& c);
+void fields_reduce_windows(Ctx& c, bool do_rho, bool do_dens);
+void fields_get(Ctx& c, int which, int s, double* out);
+void fields_set(Ctx& c, int which, int s, const double* in);
+double fields_energy(Ctx& c);
+
+void poisson_init(Ctx& c);
+int poisson_solve(Ctx& c, double tol, int max_cycles, double* relres, const double* rhs_override);
+
+void objects_set(Ctx& c, int o, const uint8_t* mask, int ncmat, const int32_t* ijk, int fixed, double fixed_pot);
+void objects_build_cmatrix(Ctx& c, int o);
+void objects_set_cmatrix(Ctx& c, int o, const double* C);
+void objects_get_cmatrix(Ctx& c, int o, double* C);
+void objects_sum_whole_charge(Ctx& c);
+void objects_redistribute(Ctx& c);
+void objects_add_charge(Ctx& c, int o, const double* dq, const double* dcur);
+void objects_state(Ctx& c, int o, double* pot, double* q, double* cur, double* cmap);
+void objects_set_charge_map(Ctx& c, int o, const double* cmap);
+
+void timer_begin(Ctx& c, const char* name, hipEvent_t* ev);
+void timer_end(Ctx& c, const char* name, hipEvent_t ev);
+void timers_collect(Ctx& c);
+
+inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Scoped phase timing on the context stream.
+struct Phase {
+  Ctx& c;
+  const char* name;
+  hipEvent_t ev = nullptr;
+  Phase(Ctx& c_, const char* n) : c(c_), name(n) { timer_begin(c, name, &ev); }
+  ~Phase() { timer_end(c, name, ev); }
+};
+
+}  // namespace fopis

@@ -1,0 +1,363 @@
+// Conductors by the capacitance-matrix method, FP64 on the device.
+//
+// Reference behaviour followed (hsimyu/fopis):
+//   construct / cell map       src/Spacecraft/Spacecraft.cpp:16-152
+//   capacity matrix setup      src/Grid/RootGrid/RootGridObject.cpp:77-148 (B by unit-charge solves, C = B^-1)
+//   sumWholeCharge             src/Spacecraft/Spacecraft.cpp:615-626
+//   getTotalCharge             src/Spacecraft/Spacecraft.cpp:248-259
+//   perfect-conductor charge   src/Spacecraft/Spacecraft.cpp:708-758
+//   Cmatrix::total             include/Spacecraft/Cmatrix.hpp:36-44
+#include "fopis_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace fopis {
+
+namespace {
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_set_node_cmat(int32_t* node_cmat, const int64_t* cmat_node, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) node_cmat[cmat_node[i]] = i;
+}
+
+__global__ void k_unit_rhs(double* rho0, int64_t nn, int64_t node) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nn) rho0[i] = (i == node) ? 1.0 : 0.0;
+}
+
+__global__ void k_zero_d(double* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// B(col,row) = phi at the row node for the unit charge at the col node
+__global__ void k_gather_col(double* B, int n, int col, const double* phi, const int64_t* cmat_node) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row < n) B[(size_t)col * n + row] = phi[cmat_node[row]];
+}
+
+__global__ void k_identity(double* A, int n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (int64_t)n * n) A[i] = (i / n == i % n) ? 1.0 : 0.0;
+}
+
+// Gauss-Jordan with partial pivoting, step c: pivot search (one block).
+__global__ __launch_bounds__(1024) void k_gj_pivot(const double* A, int n, int c, int* piv, double* pval) {
+  __shared__ double bv[1024];
+  __shared__ int bi[1024];
+  double best = -1.0;
+  int arg = c;
+  for (int r = c + threadIdx.x; r < n; r += 1024) {
+    const double v = fabs(A[(size_t)r * n + c]);
+    if (v > best) {
+      best = v;
+      arg = r;
+    }
+  }
+  bv[threadIdx.x] = best;
+  bi[threadIdx.x] = arg;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double v2 = bv[threadIdx.x + o];
+      const int i2 = bi[threadIdx.x + o];
+      // largest magnitude, first row on ties (a sequential scan's choice)
+      if (v2 > bv[threadIdx.x] || (v2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) {
+        bv[threadIdx.x] = v2;
+        bi[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *piv = bi[0];
+    *pval = A[(size_t)bi[0] * n + c];
+  }
+}
+
+// swap rows c and piv, then divide row c by the pivot value
+__global__ void k_gj_swap_scale(double* A, double* I, int n, int c, const int* piv, const double* pval) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int p = *piv;
+  const double d = *pval;
+  double a = A[(size_t)p * n + q], b = I[(size_t)p * n + q];
+  if (p != c) {
+    A[(size_t)p * n + q] = A[(size_t)c * n + q];
+    I[(size_t)p * n + q] = I[(size_t)c * n + q];
+  }
+  A[(size_t)c * n + q] = a / d;
+  I[(size_t)c * n + q] = b / d;
+}
+
+__global__ void k_gj_getcol(const double* A, int n, int c, double* f) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) f[r] = A[(size_t)r * n + c];
+}
+
+// every other row r: row_r -= f[r] * row_c (rows with f == 0 unchanged)
+__global__ void k_gj_elim(double* A, double* I, int n, int c, const double* f) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (q >= n || r == c) return;
+  const double fr = f[r];
+  if (fr == 0.0) return;
+  A[(size_t)r * n + q] -= fr * A[(size_t)c * n + q];
+  I[(size_t)r * n + q] -= fr * I[(size_t)c * n + q];
+}
+
+// colsum[j] = sum_i C(i,j) (i ascending) and the Cmatrix::total order
+__global__ void k_colsum(const double* C, int n, double* colsum) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += C[(size_t)i * n + j];
+  colsum[j] = s;
+}
+
+__global__ void k_total(const double* C, int64_t nn, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double r = 0.0;
+    for (int64_t i = 0; i < nn; ++i) r += C[i];
+    *out = r;
+  }
+}
+
+// sumWholeCharge: charge_map += tmp; tmp = 0
+__global__ void k_sum_whole(double* cmap, double* tmp, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cmap[i] += tmp[i];
+  tmp[i] = 0.0;
+}
+
+// Scalars of the redistribution in one block: Q (getTotalCharge), the sum
+// over C(i,j) phi_j (by column sums) and the conductor potential V.
+__global__ __launch_bounds__(256) void k_obj_scalars(const double* cmap, int S, int n, const double* colsum,
+                                                     const double* phi, const int64_t* cmat_node, double total,
+                                                     int fixed, double fixed_pot, double* scal, double* phis) {
+  __shared__ double sq[256], sc[256];
+  double q = 0.0, ct = 0.0;
+  for (int j = threadIdx.x; j < n; j += 256) {
+    for (int s = 0; s < S; ++s) q += cmap[(size_t)s * n + j];
+    const double pj = phi[cmat_node[j]];
+    phis[j] = pj;
+    ct += colsum[j] * pj;
+  }
+  sq[threadIdx.x] = q;
+  sc[threadIdx.x] = ct;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sq[threadIdx.x] += sq[threadIdx.x + o];
+      sc[threadIdx.x] += sc[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double Q = sq[0], ctp = sc[0];
+    scal[0] = fixed ? fixed_pot : (ctp + Q) / total;
+    scal[1] = Q;
+    scal[2] = ctp;
+  }
+}
+
+// delta_rho_i = sum_j C(i,j) (V - phi_j), one wavefront per row; added to rho[0]
+__global__ __launch_bounds__(256) void k_obj_drho(const double* C, int n, const double* phis, const double* scal,
+                                                  const int64_t* cmat_node, double* rho0) {
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (w >= n) return;
+  const double V = scal[0];
+  const double* row = C + (size_t)w * n;
+  double acc = 0.0;
+  for (int j = lane; j < n; j += 64) acc += row[j] * (V - phis[j]);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if (lane == 0) rho0[cmat_node[w]] += acc;
+}
+
+}  // namespace
+
+void objects_set(Ctx& c, int o, const uint8_t* mask, int ncmat, const int32_t* ijk, int fixed, double fixed_pot) {
+  if (o < 0 || o >= FOPIS_MAX_OBJECTS) throw Error(FOPIS_E_ARG, "object index out of range");
+  if (ncmat < 0) throw Error(FOPIS_E_ARG, "negative ncmat");
+  ObjectDev& ob = c.obj[o];
+  hipStream_t st = c.stream;
+  const size_t nm = c.ncell_d;
+  // merge this object's cells into the shared mask (value o+1)
+  std::vector<uint8_t> hm(nm, 0);
+  if (c.have_mask) FOPIS_HIP(hipMemcpy(hm.data(), c.mask.p, nm, hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < nm; ++q) {
+    if (hm[q] == (uint8_t)(o + 1)) hm[q] = 0;
+    if (mask[q]) {
+      if (hm[q] != 0) throw Error(FOPIS_E_ARG, "objects overlap");
+      hm[q] = (uint8_t)(o + 1);
+    }
+  }
+  c.mask.alloc(nm);
+  FOPIS_HIP(hipMemcpy(c.mask.p, hm.data(), nm, hipMemcpyHostToDevice));
+  c.have_mask = true;
+  ob.ncmat = ncmat;
+  ob.fixed = fixed != 0;
+  ob.fixed_potential = fixed_pot;
+  ob.defined = ncmat > 0;
+  ob.cmat_ijk.assign(ijk, ijk + 3 * (size_t)ncmat);
+  std::vector<int64_t> nodes(ncmat);
+  for (int q = 0; q < ncmat; ++q) {
+    const int i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
+    if (i < 0 || j < 0 || k < 0 || i >= c.cx || j >= c.cy || k >= c.cz)
+      throw Error(FOPIS_E_ARG, "cmat node outside the grid");
+    nodes[q] = (int64_t)c.id3(i, j, k);
+  }
+  ob.node_cmat.alloc(c.nnode);
+  k_fill_i32<<<div_up((int64_t)c.nnode, 256), 256, 0, st>>>(ob.node_cmat.p, (int64_t)c.nnode, -1);
+  ob.cmat_node.alloc(std::max(ncmat, 1));
+  if (ncmat > 0) {
+    FOPIS_HIP(hipMemcpyAsync(ob.cmat_node.p, nodes.data(), ncmat * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    k_set_node_cmat<<<div_up(ncmat, 256), 256, 0, st>>>(ob.node_cmat.p, ob.cmat_node.p, ncmat);
+  }
+  ob.C.alloc(std::max<size_t>((size_t)ncmat * ncmat, 1));
+  ob.colsum.alloc(std::max(ncmat, 1));
+  ob.tmp_charge.alloc(std::max<size_t>((size_t)c.S * ncmat, 1));
+  ob.charge_map.alloc(std::max<size_t>((size_t)c.S * ncmat, 1));
+  ob.absorbed.alloc(c.S);
+  ob.scal.alloc(4 + std::max(ncmat, 1));
+  FOPIS_HIP(hipMemsetAsync(ob.C.p, 0, ob.C.n * sizeof(double), st));
+  FOPIS_HIP(hipMemsetAsync(ob.tmp_charge.p, 0, ob.tmp_charge.n * sizeof(double), st));
+  FOPIS_HIP(hipMemsetAsync(ob.charge_map.p, 0, ob.charge_map.n * sizeof(double), st));
+  FOPIS_HIP(hipMemsetAsync(ob.absorbed.p, 0, c.S * sizeof(unsigned long long), st));
+  FOPIS_HIP(hipMemsetAsync(ob.scal.p, 0, ob.scal.n * sizeof(double), st));
+  ob.current.assign(c.S, 0.0);
+  ob.host_current_add.assign(c.S, 0.0);
+  ob.total_cmat = 0.0;
+  c.nobj = std::max(c.nobj, o + 1);
+  FOPIS_HIP(hipStreamSynchronize(st));
+}
+
+static void finish_cmatrix(Ctx& c, ObjectDev& ob) {
+  const int n = ob.ncmat;
+  k_colsum<<<div_up(n, 256), 256, 0, c.stream>>>(ob.C.p, n, ob.colsum.p);
+  k_total<<<1, 1, 0, c.stream>>>(ob.C.p, (int64_t)n * n, ob.scal.p + 3);
+  FOPIS_HIP(hipMemcpyAsync(&ob.total_cmat, ob.scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+}
+
+void objects_build_cmatrix(Ctx& c, int o) {
+  ObjectDev& ob = c.obj[o];
+  const int n = ob.ncmat;
+  if (n <= 0) return;
+  hipStream_t st = c.stream;
+  const int64_t nn = (int64_t)c.nnode;
+  std::vector<int64_t> nodes(n);
+  FOPIS_HIP(hipMemcpy(nodes.data(), ob.cmat_node.p, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+  DBuf<double> B;
+  B.alloc((size_t)n * n);
+  k_zero_d<<<div_up(nn, 256), 256, 0, st>>>(c.phi.p, nn);
+  for (int col = 0; col < n; ++col) {
+    k_unit_rhs<<<div_up(nn, 256), 256, 0, st>>>(c.rho.p, nn, nodes[col]);
+    double rr = 0.0;
+    poisson_solve(c, 1e-11, 400, &rr, c.rho.p);
+    k_gather_col<<<div_up(n, 256), 256, 0, st>>>(B.p, n, col, c.phi.p, ob.cmat_node.p);
+  }
+  k_zero_d<<<div_up(nn, 256), 256, 0, st>>>(c.rho.p, nn);
+  // C = B^-1 by Gauss-Jordan with partial pivoting
+  k_identity<<<div_up((int64_t)n * n, 256), 256, 0, st>>>(ob.C.p, n);
+  DBuf<double> f;
+  f.alloc(n + 2);
+  DBuf<int> piv;
+  piv.alloc(1);
+  for (int col = 0; col < n; ++col) {
+    k_gj_pivot<<<1, 1024, 0, st>>>(B.p, n, col, piv.p, f.p + n);
+    k_gj_swap_scale<<<div_up(n, 256), 256, 0, st>>>(B.p, ob.C.p, n, col, piv.p, f.p + n);
+    k_gj_getcol<<<div_up(n, 256), 256, 0, st>>>(B.p, n, col, f.p);
+    k_gj_elim<<<dim3(div_up(n, 256), n), 256, 0, st>>>(B.p, ob.C.p, n, col, f.p);
+  }
+  FOPIS_HIP(hipGetLastError());
+  finish_cmatrix(c, ob);
+}
+
+void objects_set_cmatrix(Ctx& c, int o, const double* C) {
+  ObjectDev& ob = c.obj[o];
+  const size_t n2 = (size_t)ob.ncmat * ob.ncmat;
+  if (n2 == 0) return;
+  FOPIS_HIP(hipMemcpy(ob.C.p, C, n2 * sizeof(double), hipMemcpyHostToDevice));
+  finish_cmatrix(c, ob);
+}
+
+void objects_get_cmatrix(Ctx& c, int o, double* C) {
+  ObjectDev& ob = c.obj[o];
+  const size_t n2 = (size_t)ob.ncmat * ob.ncmat;
+  if (n2 == 0) return;
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  FOPIS_HIP(hipMemcpy(C, ob.C.p, n2 * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void objects_sum_whole_charge(Ctx& c) {
+  for (int o = 0; o < c.nobj; ++o) {
+    ObjectDev& ob = c.obj[o];
+    if (!ob.defined) continue;
+    const int64_t n = (int64_t)c.S * ob.ncmat;
+    k_sum_whole<<<div_up(n, 256), 256, 0, c.stream>>>(ob.charge_map.p, ob.tmp_charge.p, n);
+  }
+  FOPIS_HIP(hipGetLastError());
+}
+
+void objects_redistribute(Ctx& c) {
+  for (int o = 0; o < c.nobj; ++o) {
+    ObjectDev& ob = c.obj[o];
+    if (!ob.defined) continue;
+    const int n = ob.ncmat;
+    if (ob.total_cmat == 0.0 && !ob.fixed) throw Error(FOPIS_E_STATE, "capacity matrix not set");
+    k_obj_scalars<<<1, 256, 0, c.stream>>>(ob.charge_map.p, c.S, n, ob.colsum.p, c.phi.p, ob.cmat_node.p,
+                                           ob.total_cmat, ob.fixed ? 1 : 0, ob.fixed_potential, ob.scal.p,
+                                           ob.scal.p + 4);
+    k_obj_drho<<<div_up((int64_t)n * 64, 256), 256, 0, c.stream>>>(ob.C.p, n, ob.scal.p + 4, ob.scal.p,
+                                                                   ob.cmat_node.p, c.rho.p);
+  }
+  FOPIS_HIP(hipGetLastError());
+}
+
+void objects_add_charge(Ctx& c, int o, const double* dq, const double* dcur) {
+  ObjectDev& ob = c.obj[o];
+  const size_t n = (size_t)c.S * ob.ncmat;
+  if (dq && n > 0) {
+    std::vector<double> cur(n);
+    FOPIS_HIP(hipStreamSynchronize(c.stream));
+    FOPIS_HIP(hipMemcpy(cur.data(), ob.tmp_charge.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < n; ++q) cur[q] += dq[q];
+    FOPIS_HIP(hipMemcpy(ob.tmp_charge.p, cur.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (dcur)
+    for (int s = 0; s < c.S; ++s) ob.host_current_add[s] += dcur[s];
+}
+
+void objects_state(Ctx& c, int o, double* pot, double* q, double* cur, double* cmap) {
+  ObjectDev& ob = c.obj[o];
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  double sc[4] = {0, 0, 0, 0};
+  FOPIS_HIP(hipMemcpy(sc, ob.scal.p, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  if (pot) *pot = sc[0];
+  if (q) *q = sc[1];
+  if (cur) {
+    std::vector<unsigned long long> ab(c.S);
+    FOPIS_HIP(hipMemcpy(ab.data(), ob.absorbed.p, c.S * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int s = 0; s < c.S; ++s)
+      cur[s] = (double)ab[s] * (c.d.species[s].charge * c.d.species[s].size) + ob.host_current_add[s];
+  }
+  if (cmap && ob.ncmat > 0)
+    FOPIS_HIP(hipMemcpy(cmap, ob.charge_map.p, (size_t)c.S * ob.ncmat * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void objects_set_charge_map(Ctx& c, int o, const double* cmap) {
+  ObjectDev& ob = c.obj[o];
+  if (ob.ncmat <= 0) return;
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  FOPIS_HIP(hipMemcpy(ob.charge_map.p, cmap, (size_t)c.S * ob.ncmat * sizeof(double), hipMemcpyHostToDevice));
+}
+
+}  // namespace fopis

@@ -1,0 +1,1232 @@
+// Particle hot path on gfx950: tile-binned SoA particles, the fused
+// gather-Boris-move-absorb-count-deposit-bin kernel, arrivals, re-binning and
+// particle reductions.
+//
+// Reference behaviour followed (hsimyu/fopis):
+//   position/weights  include/position.hpp:13-21,59-69
+//   velocity (Boris)  src/Grid/GridParticleTreatmentES.cpp:13-88
+//   position + absorb src/Grid/GridParticleTreatmentES.cpp:90-113, include/grid.hpp:309-395
+//   density           src/Grid/RootGrid/RootGridField.cpp:1129-1150
+//   CIC deposit       src/Grid/RootGrid/RootGridField.cpp:1161-1192
+//   conductor capture src/Spacecraft/Spacecraft.cpp:512-613, src/Particle/particle.cpp:46-204
+//   energy            src/Grid/GridParticleTreatment.cpp:104-118
+//
+// FP64 arithmetic is written in the reference's operation order and compiled
+// without contraction, so an FP64 push reproduces the reference bit for bit
+// on identical inputs.
+#include "fopis_internal.hpp"
+#include "radix.hpp"
+
+#pragma clang fp contract(off)
+
+namespace fopis {
+
+enum : int { M_VEL = 1, M_POS = 2, M_DEP = 4, M_BIN = 8, M_DENS = 16 };
+
+struct ObjArgs {
+  const int32_t* node_cmat;      // [(nx+2)(ny+2)(nz+2)] -> cmat index or -1
+  double* tmp;                   // temporary charge map of this species
+  unsigned long long* absorbed;  // absorbed superparticles of this species
+};
+
+template <class R>
+struct TileArgs {
+  int N[3];        // cells per axis
+  int T[3];        // tiles per axis
+  int cx, cy, cz;  // node array dims with glue
+  double xmax[3];  // largest valid coordinate, n - 1 (slx - dx)
+  double qm, q, bx, by, bz, boris, size;
+  const double *ex, *ey, *ez;
+  SoaPtr<R> reg;
+  const int64_t* start;
+  const int32_t* capa;
+  int32_t* cnt;
+  SoaPtr<R> out;
+  uint8_t* odir;
+  const int64_t* ostart;
+  const int32_t* ocapa;
+  int32_t* ocnt;
+  const double* in[6];
+  const int32_t* ibeg;
+  SoaPtr<R> stray;
+  int64_t stray_cap;
+  int32_t* status;  // [0] strays, [1] lost, [2] out-of-window deposits
+  R* rho_part;
+  int32_t* dens_part;
+  double* rho_g;
+  double* dens_g;
+  const uint8_t* mask;
+  int nobj;
+  ObjArgs obj[FOPIS_MAX_OBJECTS];
+};
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ int fdiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+struct DPos {  // include/position.hpp in FP64
+  int i, j, k;
+  double x, y, z, dx1, dy1, dz1, dx2, dy2, dz2;
+};
+
+__device__ __forceinline__ void set_xyz(DPos& p, double x, double y, double z) {
+  p.x = x;
+  p.y = y;
+  p.z = z;
+  p.i = (int)floor(x) + 1;
+  p.j = (int)floor(y) + 1;
+  p.k = (int)floor(z) + 1;
+  p.dx1 = x - (double)(p.i - 1);
+  p.dy1 = y - (double)(p.j - 1);
+  p.dz1 = z - (double)(p.k - 1);
+  p.dx2 = 1.0 - p.dx1;
+  p.dy2 = 1.0 - p.dy1;
+  p.dz2 = 1.0 - p.dz1;
+}
+
+template <class R>
+struct Cond {
+  const TileArgs<R>& a;
+  int o;
+  __device__ bool valid_cell(int i, int j, int k) const {
+    return i >= 0 && i < a.cx - 1 && j >= 0 && j < a.cy - 1 && k >= 0 && k < a.cz - 1;
+  }
+  __device__ bool cell(int i, int j, int k) const {
+    if (!valid_cell(i, j, k)) return false;
+    return a.mask[((size_t)i * (a.cy - 1) + j) * (a.cz - 1) + k] == (uint8_t)(o + 1);
+  }
+  __device__ int cmat(int i, int j, int k) const {
+    if (i < 0 || j < 0 || k < 0 || i >= a.cx || j >= a.cy || k >= a.cz) return -1;
+    return a.obj[o].node_cmat[((size_t)i * a.cy + j) * a.cz + k];
+  }
+  // Spacecraft.cpp:461-504
+  __device__ bool xpoint(const DPos& p, int sign) const {
+    if (p.dx1 >= 1e-10) return false;
+    if (valid_cell(p.i, p.j, p.k) && valid_cell(p.i - 1, p.j, p.k))
+      return sign > 0 ? (cell(p.i, p.j, p.k) && !cell(p.i - 1, p.j, p.k))
+                      : (!cell(p.i, p.j, p.k) && cell(p.i - 1, p.j, p.k));
+    return cmat(p.i, p.j, p.k) >= 0 && cmat(p.i, p.j + 1, p.k) >= 0 && cmat(p.i, p.j, p.k + 1) >= 0 &&
+           cmat(p.i, p.j + 1, p.k + 1) >= 0;
+  }
+  __device__ bool ypoint(const DPos& p, int sign) const {
+    if (p.dy1 >= 1e-10) return false;
+    if (valid_cell(p.i, p.j, p.k) && valid_cell(p.i, p.j - 1, p.k))
+      return sign > 0 ? (cell(p.i, p.j, p.k) && !cell(p.i, p.j - 1, p.k))
+                      : (!cell(p.i, p.j, p.k) && cell(p.i, p.j - 1, p.k));
+    return cmat(p.i, p.j, p.k) >= 0 && cmat(p.i + 1, p.j, p.k) >= 0 && cmat(p.i, p.j, p.k + 1) >= 0 &&
+           cmat(p.i + 1, p.j, p.k + 1) >= 0;
+  }
+  __device__ bool zpoint(const DPos& p, int sign) const {
+    if (p.dz1 >= 1e-10) return false;
+    if (valid_cell(p.i, p.j, p.k) && valid_cell(p.i, p.j, p.k - 1))
+      return sign > 0 ? (cell(p.i, p.j, p.k) && !cell(p.i, p.j, p.k - 1))
+                      : (!cell(p.i, p.j, p.k) && cell(p.i, p.j, p.k - 1));
+    return cmat(p.i, p.j, p.k) >= 0 && cmat(p.i + 1, p.j, p.k) >= 0 && cmat(p.i, p.j + 1, p.k) >= 0 &&
+           cmat(p.i + 1, p.j + 1, p.k) >= 0;
+  }
+  __device__ void add(int c, double v) const {
+    if (c >= 0) atomicAdd(&a.obj[o].tmp[c], v);
+  }
+  // Spacecraft.cpp:594-613
+  __device__ void addx(const DPos& p) const {
+    add(cmat(p.i, p.j, p.k), a.q * p.dy2 * p.dz2);
+    add(cmat(p.i, p.j + 1, p.k), a.q * p.dy1 * p.dz2);
+    add(cmat(p.i, p.j, p.k + 1), a.q * p.dy2 * p.dz1);
+    add(cmat(p.i, p.j + 1, p.k + 1), a.q * p.dy1 * p.dz1);
+  }
+  __device__ void addy(const DPos& p) const {
+    add(cmat(p.i, p.j, p.k), a.q * p.dx2 * p.dz2);
+    add(cmat(p.i + 1, p.j, p.k), a.q * p.dx1 * p.dz2);
+    add(cmat(p.i, p.j, p.k + 1), a.q * p.dx2 * p.dz1);
+    add(cmat(p.i + 1, p.j, p.k + 1), a.q * p.dx1 * p.dz1);
+  }
+  __device__ void addz(const DPos& p) const {
+    add(cmat(p.i, p.j, p.k), a.q * p.dx2 * p.dy2);
+    add(cmat(p.i + 1, p.j, p.k), a.q * p.dx1 * p.dy2);
+    add(cmat(p.i, p.j + 1, p.k), a.q * p.dx2 * p.dy1);
+    add(cmat(p.i + 1, p.j + 1, p.k), a.q * p.dx1 * p.dy1);
+  }
+};
+
+__device__ __forceinline__ double move_ratio(double d1, double d2, double v) {
+  if (v == 0.0) return 0.0;
+  return v > 0.0 ? d1 / fabs(v) : d2 / fabs(v);
+}
+
+// Spacecraft::distributeInnerParticleCharge with Particle::computeCrossPoints
+// (particle.cpp:82-204): the charge goes to the four cmat nodes of the first
+// crossed face whose orientation matches the velocity sign.
+template <class R>
+__device__ void absorb(const Cond<R>& cd, double x, double y, double z, double vx, double vy, double vz) {
+  DPos pos, old;
+  set_xyz(pos, x, y, z);
+  set_xyz(old, x - vx, y - vy, z - vz);
+  const double mvx = 1.0 - move_ratio(pos.dx1, pos.dx2, vx);
+  const double mvy = 1.0 - move_ratio(pos.dy1, pos.dy2, vy);
+  const double mvz = 1.0 - move_ratio(pos.dz1, pos.dz2, vz);
+  const bool mx = pos.i != old.i, my = pos.j != old.j, mz = pos.k != old.k;
+  DPos xc{}, yc{}, zc{};
+  if (mx) set_xyz(xc, vx > 0.0 ? ceil(old.x) : floor(old.x), old.y + vy * mvx, old.z + vz * mvx);
+  if (my) set_xyz(yc, old.x + vx * mvy, vy > 0.0 ? ceil(old.y) : floor(old.y), old.z + vz * mvy);
+  if (mz) set_xyz(zc, old.x + vx * mvz, old.y + vy * mvz, vz > 0.0 ? ceil(old.z) : floor(old.z));
+  // order codes: 0 = x, 1 = y, 2 = z
+  int ord[3];
+  int n = 0;
+  const int count = (int)mx + (int)my + (int)mz;
+  if (count == 3) {
+    if (mvx <= mvy) {
+      if (mvx <= mvz) {
+        ord[0] = 0;
+        if (mvy <= mvz) {
+          ord[1] = 1;
+          ord[2] = 2;
+        } else {
+          ord[1] = 2;
+          ord[2] = 1;
+        }
+      } else {
+        ord[0] = 2;
+        ord[1] = 0;
+        ord[2] = 1;
+      }
+    } else {
+      if (mvy <= mvz) {
+        ord[0] = 1;
+        if (mvx <= mvz) {
+          ord[1] = 0;
+          ord[2] = 2;
+        } else {
+          ord[1] = 2;
+          ord[2] = 0;
+        }
+      } else {
+        ord[0] = 2;
+        ord[1] = 1;
+        ord[2] = 0;
+      }
+    }
+    n = 3;
+  } else if (count == 2) {
+    if (mx) {
+      if (my) {
+        ord[0] = mvx <= mvy ? 0 : 1;
+        ord[1] = mvx <= mvy ? 1 : 0;
+      } else {
+        ord[0] = 0;  // particle.cpp:175 compares mvx with itself
+        ord[1] = 2;
+      }
+    } else {
+      ord[0] = mvy <= mvz ? 1 : 2;
+      ord[1] = mvy <= mvz ? 2 : 1;
+    }
+    n = 2;
+  } else if (count == 1) {
+    ord[0] = mx ? 0 : (my ? 1 : 2);
+    n = 1;
+  }
+  const int is = vx > 0.0 ? 1 : -1, js = vy > 0.0 ? 1 : -1, ks = vz > 0.0 ? 1 : -1;
+  for (int q = 0; q < n; ++q) {
+    const DPos& c = ord[q] == 0 ? xc : (ord[q] == 1 ? yc : zc);
+    if (cd.xpoint(c, is)) {
+      cd.addx(c);
+      break;
+    } else if (cd.ypoint(c, js)) {
+      cd.addy(c);
+      break;
+    } else if (cd.zpoint(c, ks)) {
+      cd.addz(c);
+      break;
+    }
+  }
+  atomicAdd(cd.a.obj[cd.o].absorbed, 1ull);
+}
+
+// Ranks of two predicates over the block, in thread order.
+__device__ __forceinline__ void block_rank2(bool pa, bool pb, int& ra, int& rb, int& ta, int& tb, int* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const unsigned long long ma = __ballot(pa), mb = __ballot(pb);
+  if (lane == 0) {
+    sh[w] = __popcll(ma);
+    sh[4 + w] = __popcll(mb);
+  }
+  __syncthreads();
+  int oa = 0, ob = 0;
+  ta = 0;
+  tb = 0;
+#pragma unroll
+  for (int q = 0; q < kBlock / 64; ++q) {
+    const int ca = sh[q], cb = sh[4 + q];
+    if (q < w) {
+      oa += ca;
+      ob += cb;
+    }
+    ta += ca;
+    tb += cb;
+  }
+  ra = oa + __popcll(ma & lt);
+  rb = ob + __popcll(mb & lt);
+  __syncthreads();
+}
+
+template <class R>
+__device__ __forceinline__ void put_stray(const TileArgs<R>& a, const R v[6]) {
+  const int slot = atomicAdd(&a.status[0], 1);
+  if (slot < a.stray_cap) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.stray.c[c][slot] = v[c];
+  } else {
+    atomicAdd(&a.status[1], 1);
+  }
+}
+
+template <class R, int MODE>
+struct TileSmem {
+  static constexpr bool kE = (MODE & M_VEL) != 0;
+  static constexpr bool kR = (MODE & M_DEP) != 0;
+  static constexpr bool kD = (MODE & (M_DEP | M_DENS)) != 0;
+  static constexpr size_t e_bytes = kE ? sizeof(R) * 3 * kEW * kEW * kEW : 0;
+  static constexpr size_t r_bytes = kR ? sizeof(R) * kRW * kRW * kRW : 0;
+  static constexpr size_t d_bytes = kD ? sizeof(int) * kDW * kDW * kDW : 0;
+  static constexpr size_t bytes = e_bytes + r_bytes + d_bytes + 16;
+};
+
+// One workgroup per tile: stage the tile's E nodes in LDS, stream its
+// particles (and the staged new ones) in chunks, apply the requested stages,
+// accumulate charge and counts in LDS windows, and re-bin: particles staying
+// in the tile are compacted in place, the others go to the tile's outbox.
+template <class R, int MODE>
+__global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
+  using SM = TileSmem<R, MODE>;
+  __shared__ __attribute__((aligned(16))) char smem[SM::bytes];
+  __shared__ int sh_rank[8];
+  R* sE = reinterpret_cast<R*>(smem);
+  R* sR = reinterpret_cast<R*>(smem + SM::e_bytes);
+  int* sD = reinterpret_cast<int*>(smem + SM::e_bytes + SM::r_bytes);
+
+  const int t = blockIdx.x;
+  const int tz = t % a.T[2], ty = (t / a.T[2]) % a.T[1], tx = t / (a.T[2] * a.T[1]);
+  const int o0 = tx * kTile, o1 = ty * kTile, o2 = tz * kTile;
+
+  if constexpr (SM::kE) {
+    constexpr int ne = kEW * kEW * kEW;
+    for (int q = threadIdx.x; q < ne; q += kBlock) {
+      const int aa = q / (kEW * kEW), bb = (q / kEW) % kEW, cc = q % kEW;
+      const int gi = o0 + 1 + aa, gj = o1 + 1 + bb, gk = o2 + 1 + cc;
+      R vx = 0, vy = 0, vz = 0;
+      if (gi < a.cx && gj < a.cy && gk < a.cz) {
+        const size_t g = ((size_t)gi * a.cy + gj) * a.cz + gk;
+        vx = (R)a.ex[g];
+        vy = (R)a.ey[g];
+        vz = (R)a.ez[g];
+      }
+      sE[q] = vx;
+      sE[ne + q] = vy;
+      sE[2 * ne + q] = vz;
+    }
+  }
+  if constexpr (SM::kR) {
+    for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) sR[q] = (R)0;
+  }
+  if constexpr (SM::kD) {
+    for (int q = threadIdx.x; q < kDW * kDW * kDW; q += kBlock) sD[q] = 0;
+  }
+  __syncthreads();
+
+  const int64_t base = a.start[t];
+  const int nreg = a.cnt[t];
+  int nin = 0, ib0 = 0;
+  if constexpr ((MODE & (M_DEP | M_DENS)) != 0) {
+    if (a.ibeg) {
+      ib0 = a.ibeg[t];
+      nin = a.ibeg[t + 1] - ib0;
+    }
+  }
+  const int ntot = nreg + nin;
+  int wptr = 0, optr = 0;
+  const int ocap = ((MODE & M_BIN) != 0) ? a.ocapa[t] : 0;
+  const int rcap = a.capa[t];
+
+  for (int c0 = 0; c0 < ntot; c0 += kBlock) {
+    const int idx = c0 + threadIdx.x;
+    const bool act = idx < ntot;
+    const bool from_in = idx >= nreg;
+    R p[6] = {0, 0, 0, 0, 0, 0};
+    if (act) {
+      if (!from_in) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) p[c] = a.reg.c[c][base + idx];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) p[c] = (R)a.in[c][ib0 + idx - nreg];
+      }
+    }
+    bool valid = act && !isnan(p[0]);
+
+    // ---- velocity: trilinear gather of nodal E, Boris rotation with t = b
+    if constexpr ((MODE & M_VEL) != 0) {
+      if (valid && !from_in) {
+        const R x = p[0], y = p[1], z = p[2];
+        int ci = (int)floor(x), cj = (int)floor(y), ck = (int)floor(z);
+        ci = min(ci, a.N[0] - 1);
+        cj = min(cj, a.N[1] - 1);
+        ck = min(ck, a.N[2] - 1);
+        const R dx1 = x - (R)ci, dy1 = y - (R)cj, dz1 = z - (R)ck;
+        const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
+        const R qm = (R)a.qm;
+        const R v1 = qm * dx2 * dy2 * dz2;
+        const R v2 = qm * dx1 * dy2 * dz2;
+        const R v3 = qm * dx2 * dy1 * dz2;
+        const R v4 = qm * dx1 * dy1 * dz2;
+        const R v5 = qm * dx2 * dy2 * dz1;
+        const R v6 = qm * dx1 * dy2 * dz1;
+        const R v7 = qm * dx2 * dy1 * dz1;
+        const R v8 = qm * dx1 * dy1 * dz1;
+        const int li = ci - o0, lj = cj - o1, lk = ck - o2;
+        R e[3];
+        if (li >= 0 && li < kTile && lj >= 0 && lj < kTile && lk >= 0 && lk < kTile) {
+          constexpr int ne = kEW * kEW * kEW;
+          const int b0 = (li * kEW + lj) * kEW + lk;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const R* E = sE + c * ne + b0;
+            e[c] = v1 * E[0] + v2 * E[kEW * kEW] + v3 * E[kEW] + v4 * E[kEW * kEW + kEW] + v5 * E[1] +
+                   v6 * E[kEW * kEW + 1] + v7 * E[kEW + 1] + v8 * E[kEW * kEW + kEW + 1];
+          }
+        } else {  // outside the tile (a bin left stale by an interrupted step): global gather
+          const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
+          const size_t g = (size_t)(ci + 1) * sI + (size_t)(cj + 1) * sJ + (ck + 1);
+          const double* Es[3] = {a.ex, a.ey, a.ez};
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double* E = Es[c] + g;
+            e[c] = v1 * (R)E[0] + v2 * (R)E[sI] + v3 * (R)E[sJ] + v4 * (R)E[sI + sJ] + v5 * (R)E[1] +
+                   v6 * (R)E[sI + 1] + v7 * (R)E[sJ + 1] + v8 * (R)E[sI + sJ + 1];
+          }
+        }
+        const R bx = (R)a.bx, by = (R)a.by, bz = (R)a.bz, boris = (R)a.boris;
+        const R vx1 = p[3] + e[0], vy1 = p[4] + e[1], vz1 = p[5] + e[2];
+        const R vxt = vx1 + vy1 * bz - vz1 * by;
+        const R vyt = vy1 + vz1 * bx - vx1 * bz;
+        const R vzt = vz1 + vx1 * by - vy1 * bx;
+        p[3] = vx1 + (e[0] + boris * (vyt * bz - vzt * by)) * (R)1;
+        p[4] = vy1 + (e[1] + boris * (vzt * bx - vxt * bz)) * (R)1;
+        p[5] = vz1 + (e[2] + boris * (vxt * by - vyt * bx)) * (R)1;
+      }
+    }
+
+    // ---- position: x += v; leaving a physical face invalidates
+    if constexpr ((MODE & M_POS) != 0) {
+      if (valid && !from_in) {
+        p[0] = p[0] + p[3];
+        p[1] = p[1] + p[4];
+        p[2] = p[2] + p[5];
+        if (p[0] < (R)0 || p[0] > (R)a.xmax[0] || p[1] < (R)0 || p[1] > (R)a.xmax[1] || p[2] < (R)0 ||
+            p[2] > (R)a.xmax[2])
+          valid = false;
+      }
+    }
+
+    int ci = 0, cj = 0, ck = 0;
+    if (valid) {
+      ci = (int)floor(p[0]);
+      cj = (int)floor(p[1]);
+      ck = (int)floor(p[2]);
+      if (from_in && (p[0] < (R)0 || p[0] > (R)a.xmax[0] || p[1] < (R)0 || p[1] > (R)a.xmax[1] ||
+                      p[2] < (R)0 || p[2] > (R)a.xmax[2]))
+        valid = false;
+    }
+
+    // ---- density: count per reference cell (RootGridField.cpp:1129-1150)
+    if constexpr (SM::kD) {
+      if (valid) {
+        const int wi = ci - o0 + 1, wj = cj - o1 + 1, wk = ck - o2 + 1;
+        if (wi >= 0 && wi < kDW && wj >= 0 && wj < kDW && wk >= 0 && wk < kDW) {
+          atomicAdd(&sD[(wi * kDW + wj) * kDW + wk], 1);
+        } else {
+          const size_t g = ((size_t)(ci + 1) * (a.cy - 1) + (cj + 1)) * (a.cz - 1) + (ck + 1);
+          atomicAdd(&a.dens_g[g], a.size);
+          atomicAdd(&a.status[2], 1);
+        }
+      }
+    }
+
+    // ---- conductor capture, then CIC deposit
+    if constexpr ((MODE & M_DEP) != 0) {
+      if (valid && a.nobj > 0) {
+        const int mi = ci + 1, mj = cj + 1, mk = ck + 1;
+        const uint8_t m = a.mask[((size_t)mi * (a.cy - 1) + mj) * (a.cz - 1) + mk];
+        if (m != 0) {
+          Cond<R> cd{a, (int)m - 1};
+          absorb(cd, (double)p[0], (double)p[1], (double)p[2], (double)p[3], (double)p[4], (double)p[5]);
+          valid = false;
+        }
+      }
+      if (valid) {
+        const int di = min(ci, a.N[0] - 1), dj = min(cj, a.N[1] - 1), dk = min(ck, a.N[2] - 1);
+        const R dx1 = p[0] - (R)di, dy1 = p[1] - (R)dj, dz1 = p[2] - (R)dk;
+        const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
+        const R q = (R)a.q;
+        const R w[8] = {dx2 * dy2 * dz2 * q, dx1 * dy2 * dz2 * q, dx2 * dy1 * dz2 * q, dx1 * dy1 * dz2 * q,
+                        dx2 * dy2 * dz1 * q, dx1 * dy2 * dz1 * q, dx2 * dy1 * dz1 * q, dx1 * dy1 * dz1 * q};
+        const int wi = di - o0 + 1, wj = dj - o1 + 1, wk = dk - o2 + 1;
+        if (wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW) {
+          R* W = sR + (wi * kRW + wj) * kRW + wk;
+          atomicAdd(W, w[0]);
+          atomicAdd(W + kRW * kRW, w[1]);
+          atomicAdd(W + kRW, w[2]);
+          atomicAdd(W + kRW * kRW + kRW, w[3]);
+          atomicAdd(W + 1, w[4]);
+          atomicAdd(W + kRW * kRW + 1, w[5]);
+          atomicAdd(W + kRW + 1, w[6]);
+          atomicAdd(W + kRW * kRW + kRW + 1, w[7]);
+        } else {
+          const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
+          double* G = a.rho_g + (size_t)(di + 1) * sI + (size_t)(dj + 1) * sJ + (dk + 1);
+          atomicAdd(G, (double)w[0]);
+          atomicAdd(G + sI, (double)w[1]);
+          atomicAdd(G + sJ, (double)w[2]);
+          atomicAdd(G + sI + sJ, (double)w[3]);
+          atomicAdd(G + 1, (double)w[4]);
+          atomicAdd(G + sI + 1, (double)w[5]);
+          atomicAdd(G + sJ + 1, (double)w[6]);
+          atomicAdd(G + sI + sJ + 1, (double)w[7]);
+          atomicAdd(&a.status[2], 1);
+        }
+      }
+    }
+
+    if constexpr ((MODE & M_BIN) != 0) {
+      int dir = 13;
+      bool far = false;
+      if (valid) {
+        const int bi = min(ci, a.N[0] - 1) / kTile, bj = min(cj, a.N[1] - 1) / kTile,
+                  bk = min(ck, a.N[2] - 1) / kTile;
+        const int d0 = bi - tx, d1 = bj - ty, d2 = bk - tz;
+        if (d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1)
+          far = true;
+        else
+          dir = (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1);
+      }
+      const bool stay = valid && !far && dir == 13;
+      const bool move = valid && !far && dir != 13;
+      int rs, rm, ts, tm;
+      block_rank2(stay, move, rs, rm, ts, tm, sh_rank);  // also orders all reads before the writes
+      if (stay) {
+        const int pos = wptr + rs;
+        if (pos < rcap) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) a.reg.c[c][base + pos] = p[c];
+        } else {
+          put_stray(a, p);
+        }
+      } else if (move) {
+        const int pos = optr + rm;
+        if (pos < ocap) {
+          const int64_t ob = a.ostart[t] + pos;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) a.out.c[c][ob] = p[c];
+          a.odir[ob] = (uint8_t)dir;
+        } else {
+          put_stray(a, p);
+        }
+      } else if (far) {
+        put_stray(a, p);
+      }
+      wptr = min(wptr + ts, rcap);
+      optr = min(optr + tm, ocap);
+    } else if constexpr ((MODE & (M_VEL | M_POS)) != 0) {
+      if (act && !from_in) {
+        if (!valid) p[0] = (R)NAN;  // invalid until the next re-bin drops it
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a.reg.c[c][base + idx] = p[c];
+      }
+    }
+  }
+
+  if constexpr ((MODE & M_BIN) != 0) {
+    if (threadIdx.x == 0) {
+      a.cnt[t] = wptr;
+      a.ocnt[t] = optr;
+    }
+  }
+  if constexpr (SM::kR || SM::kD) __syncthreads();
+  if constexpr (SM::kR) {
+    R* P = a.rho_part + (size_t)t * (kRW * kRW * kRW);
+    for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) P[q] = sR[q];
+  }
+  if constexpr (SM::kD) {
+    int32_t* P = a.dens_part + (size_t)t * (kDW * kDW * kDW);
+    for (int q = threadIdx.x; q < kDW * kDW * kDW; q += kBlock) P[q] = sD[q];
+  }
+}
+
+// Arrivals: tile t appends the outbox entries its 26 neighbours addressed to
+// it, neighbour by neighbour in direction order, each in outbox order.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_arrive(TileArgs<R> a) {
+  __shared__ int sh_rank[8];
+  const int t = blockIdx.x;
+  const int tz = t % a.T[2], ty = (t / a.T[2]) % a.T[1], tx = t / (a.T[2] * a.T[1]);
+  int n = a.cnt[t];
+  const int cap = a.capa[t];
+  const int64_t base = a.start[t];
+  for (int d = 0; d < 27; ++d) {
+    if (d == 13) continue;
+    const int sx = tx - (d / 9 - 1), sy = ty - ((d / 3) % 3 - 1), sz = tz - (d % 3 - 1);
+    if (sx < 0 || sy < 0 || sz < 0 || sx >= a.T[0] || sy >= a.T[1] || sz >= a.T[2]) continue;
+    const int s = (sx * a.T[1] + sy) * a.T[2] + sz;
+    const int m = a.ocnt[s];
+    const int64_t ob = a.ostart[s];
+    for (int c0 = 0; c0 < m; c0 += kBlock) {
+      const int i = c0 + threadIdx.x;
+      const bool hit = i < m && a.odir[ob + i] == (uint8_t)d;
+      int r, r2, tot, tot2;
+      block_rank2(hit, false, r, r2, tot, tot2, sh_rank);
+      if (hit) {
+        R p[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) p[c] = a.out.c[c][ob + i];
+        const int pos = n + r;
+        if (pos < cap) {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) a.reg.c[c][base + pos] = p[c];
+        } else {
+          put_stray(a, p);
+        }
+      }
+      n = min(n + tot, cap);
+    }
+  }
+  if (threadIdx.x == 0) a.cnt[t] = n;
+}
+
+// ------------------------------------------------------------- re-binning
+
+template <class R>
+__global__ void k_keys_tile(SoaPtr<R> f, int64_t n, TileArgs<R> a, uint32_t* keys, int with_cell, uint32_t invalid) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const R x = f.c[0][i], y = f.c[1][i], z = f.c[2][i];
+  uint32_t key = invalid;
+  const bool ok = !isnan(x) && !isnan(y) && !isnan(z) && x >= (R)0 && y >= (R)0 && z >= (R)0 &&
+                  x <= (R)a.xmax[0] && y <= (R)a.xmax[1] && z <= (R)a.xmax[2];
+  if (ok) {
+    const int ci = min((int)floor(x), a.N[0] - 1), cj = min((int)floor(y), a.N[1] - 1),
+              ck = min((int)floor(z), a.N[2] - 1);
+    const uint32_t tile = (uint32_t)(((ci / kTile) * a.T[1] + (cj / kTile)) * a.T[2] + (ck / kTile));
+    if (with_cell) {
+      const uint32_t lc = (uint32_t)(((ci % kTile) * kTile + (cj % kTile)) * kTile + (ck % kTile));
+      key = tile * (kTile * kTile * kTile) + lc;
+    } else {
+      key = tile;
+    }
+  }
+  keys[i] = key;
+}
+
+__global__ void k_count_tiles(const uint32_t* keys, int64_t n, int shift, uint32_t ntiles, uint32_t* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t t = keys[i] >> shift;
+  if (t >= ntiles) return;
+  atomicAdd(&cnt[t], 1u);
+}
+
+__global__ void k_capacity(const uint32_t* cnt, int nt, double slack, int minc, uint32_t* capa, uint32_t* ocapa) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const double want = (double)cnt[t] * slack + (double)minc;
+  uint32_t c = (uint32_t)ceil(want);
+  c = (c + 63u) & ~63u;
+  capa[t] = c;
+  uint32_t oc = c / 4u;
+  oc = (oc + 63u) & ~63u;
+  if (oc < 64u) oc = 64u;
+  ocapa[t] = oc;
+}
+
+template <class R>
+__global__ void k_fill_regions(SoaPtr<R> dst, const int64_t* start, SoaPtr<R> src, const uint32_t* keys,
+                               const uint32_t* perm, int64_t n, int shift, const int64_t* first, uint32_t ntiles) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t t = keys[i] >> shift;
+  if (t >= ntiles) return;
+  const int64_t pos = start[t] + (i - first[t]);
+  const uint32_t j = perm[i];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) dst.c[c][pos] = src.c[c][j];
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_gather_regions(SoaPtr<R> dst, const int64_t* first, SoaPtr<R> src,
+                                                           const int64_t* start, const int32_t* cnt) {
+  const int t = blockIdx.x;
+  const int64_t s0 = start[t], d0 = first[t];
+  const int n = cnt[t];
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) dst.c[c][d0 + i] = src.c[c][s0 + i];
+  }
+}
+
+template <class R>
+__global__ void k_copy_soa(SoaPtr<R> dst, int64_t doff, SoaPtr<R> src, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) dst.c[c][doff + i] = src.c[c][i];
+}
+
+template <class R>
+__global__ void k_from_f64(SoaPtr<R> dst, SoaPtr<double> src, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) dst.c[c][i] = (R)src.c[c][i];
+}
+
+template <class R>
+__global__ void k_to_f64(SoaPtr<double> dst, SoaPtr<R> src, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) dst.c[c][i] = (double)src.c[c][i];
+}
+
+__global__ void k_u32_from_i32(const int32_t* in, uint32_t* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)in[i];
+}
+
+__global__ void k_i32_from_u32(const uint32_t* in, int32_t* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int32_t)in[i];
+}
+
+template <class R>
+__global__ void k_cell_keys(SoaPtr<R> f, int64_t n, int ny1, int nz1, int32_t* keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ci = (int)floor(f.c[0][i]) + 1, cj = (int)floor(f.c[1][i]) + 1, ck = (int)floor(f.c[2][i]) + 1;
+  keys[i] = (ci * ny1 + cj) * nz1 + ck;
+}
+
+// sum of v^2 over n particles into per-block partials
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_v2(SoaPtr<R> f, int64_t n, double* part) {
+  __shared__ double s[kBlock];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double vx = f.c[3][i], vy = f.c[4][i], vz = f.c[5][i];
+    acc += vx * vx + vy * vy + vz * vz;
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void k_sum_serial(const double* part, int n, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    *out = s;
+  }
+}
+
+// ------------------------------------------------------------------ host side
+
+namespace {
+
+struct Scratch {
+  RadixSorter sorter;
+  DBuf<uint32_t> keys, perm, cnt, capa, ocapa;
+  DBuf<int64_t> first, tmp;
+};
+std::map<const Ctx*, Scratch> g_scratch;
+
+Scratch& scratch(Ctx& c) { return g_scratch[&c]; }
+
+template <class R>
+SoaPtr<R> ptrs(Soa<R>& s) {
+  SoaPtr<R> p;
+  for (int i = 0; i < 6; ++i) p.c[i] = s.c[i].p;
+  return p;
+}
+
+template <class R>
+std::vector<SpeciesBins<R>>& bins(Ctx& c);
+template <>
+std::vector<SpeciesBins<double>>& bins<double>(Ctx& c) {
+  return c.sp64;
+}
+template <>
+std::vector<SpeciesBins<float>>& bins<float>(Ctx& c) {
+  return c.sp32;
+}
+
+template <class R>
+TileArgs<R> tile_args(Ctx& c, int s) {
+  auto& b = bins<R>(c)[s];
+  TileArgs<R> a{};
+  for (int q = 0; q < 3; ++q) {
+    a.N[q] = c.Nc[q];
+    a.T[q] = c.nt[q];
+  }
+  a.cx = c.cx;
+  a.cy = c.cy;
+  a.cz = c.cz;
+  a.xmax[0] = (double)(c.nx - 1);
+  a.xmax[1] = (double)(c.ny - 1);
+  a.xmax[2] = (double)(c.nz - 1);
+  const fopis_species_desc& sd = c.d.species[s];
+  a.qm = 0.5 * sd.charge / sd.mass;
+  a.q = sd.charge * sd.size;
+  a.size = sd.size;
+  a.bx = c.d.static_b[0];
+  a.by = c.d.static_b[1];
+  a.bz = c.d.static_b[2];
+  a.boris = 2.0 / (1.0 + (a.bx * a.bx + a.by * a.by + a.bz * a.bz));
+  a.ex = c.eref.p;
+  a.ey = c.eref.p + c.nnode;
+  a.ez = c.eref.p + 2 * c.nnode;
+  a.reg = ptrs(b.reg);
+  a.start = b.start.p;
+  a.capa = b.capa.p;
+  a.cnt = b.cnt.p;
+  a.out = ptrs(b.out);
+  a.odir = b.odir.p;
+  a.ostart = b.ostart.p;
+  a.ocapa = b.ocapa.p;
+  a.ocnt = b.ocnt.p;
+  for (int q = 0; q < 6; ++q) a.in[q] = b.inbox.c[q].p;
+  a.ibeg = b.inbox_n > 0 ? b.ibeg.p : nullptr;
+  a.stray = ptrs(b.stray);
+  a.stray_cap = b.stray_cap;
+  a.status = c.status.p + 4 * s;
+  a.rho_part = reinterpret_cast<R*>(c.rho_part.p) + (size_t)s * c.ntiles * (kRW * kRW * kRW);
+  a.dens_part = c.dens_part.p + (size_t)s * c.ntiles * (kDW * kDW * kDW);
+  a.rho_g = c.rho.p + (size_t)(s + 1) * c.nnode;
+  a.dens_g = c.density.p + (size_t)s * c.ncell_d;
+  a.mask = c.mask.p;
+  a.nobj = c.have_mask ? c.nobj : 0;
+  for (int o = 0; o < c.nobj; ++o) {
+    a.obj[o].node_cmat = c.obj[o].node_cmat.p;
+    a.obj[o].tmp = c.obj[o].tmp_charge.p ? c.obj[o].tmp_charge.p + (size_t)s * c.obj[o].ncmat : nullptr;
+    a.obj[o].absorbed = c.obj[o].absorbed.p ? c.obj[o].absorbed.p + s : nullptr;
+  }
+  return a;
+}
+
+// Bins n particles of a flat set into the species' tile regions (stable by
+// tile), dropping invalid ones. key_cell additionally orders by cell inside a
+// tile and keeps the current per-tile counts.
+template <class R>
+void bin_flat(Ctx& c, int s, Soa<R>& flat, int64_t n, bool key_cell) {
+  auto& b = bins<R>(c)[s];
+  Scratch& sc = scratch(c);
+  hipStream_t st = c.stream;
+  const int nt = c.ntiles;
+  TileArgs<R> a = tile_args<R>(c, s);
+  sc.keys.alloc(std::max<size_t>(sc.keys.n, std::max<int64_t>(n, 1)));
+  sc.perm.alloc(std::max<size_t>(sc.perm.n, std::max<int64_t>(n, 1)));
+  sc.cnt.alloc(std::max<size_t>(sc.cnt.n, nt + 1));
+  sc.capa.alloc(std::max<size_t>(sc.capa.n, nt + 1));
+  sc.ocapa.alloc(std::max<size_t>(sc.ocapa.n, nt + 1));
+  sc.first.alloc(std::max<size_t>(sc.first.n, nt + 1));
+  const int shift = key_cell ? 9 : 0;
+  int bits = 1;
+  while (((int64_t)1 << bits) < ((int64_t)nt << shift) + 1) ++bits;
+  if (n > 0) {
+    // invalid particles get the key one past the last tile and sort last
+    k_keys_tile<R><<<div_up(n, 256), 256, 0, st>>>(ptrs(flat), n, a, sc.keys.p, key_cell ? 1 : 0,
+                                                   (uint32_t)nt << shift);
+    sc.sorter.sort(st, sc.keys.p, sc.perm.p, n, bits, true);
+  }
+  FOPIS_HIP(hipMemsetAsync(sc.cnt.p, 0, (nt + 1) * sizeof(uint32_t), st));
+  if (n > 0) k_count_tiles<<<div_up(n, 256), 256, 0, st>>>(sc.keys.p, n, shift, (uint32_t)nt, sc.cnt.p);
+  exclusive_scan_u32(st, sc.cnt.p, nt + 1, sc.first.p, sc.tmp);
+  if (!key_cell) {
+    k_capacity<<<div_up(nt, 256), 256, 0, st>>>(sc.cnt.p, nt, c.d.capacity_slack, 64, sc.capa.p, sc.ocapa.p);
+    b.start.alloc(nt + 1);
+    b.capa.alloc(nt + 1);
+    b.cnt.alloc(nt + 1);
+    b.ostart.alloc(nt + 1);
+    b.ocapa.alloc(nt + 1);
+    b.ocnt.alloc(nt + 1);
+    FOPIS_HIP(hipMemsetAsync(sc.capa.p + nt, 0, sizeof(uint32_t), st));
+    FOPIS_HIP(hipMemsetAsync(sc.ocapa.p + nt, 0, sizeof(uint32_t), st));
+    exclusive_scan_u32(st, sc.capa.p, nt + 1, b.start.p, sc.tmp);
+    exclusive_scan_u32(st, sc.ocapa.p, nt + 1, b.ostart.p, sc.tmp);
+    k_i32_from_u32<<<div_up(nt + 1, 256), 256, 0, st>>>(sc.capa.p, b.capa.p, nt + 1);
+    k_i32_from_u32<<<div_up(nt + 1, 256), 256, 0, st>>>(sc.ocapa.p, b.ocapa.p, nt + 1);
+    k_i32_from_u32<<<div_up(nt + 1, 256), 256, 0, st>>>(sc.cnt.p, b.cnt.p, nt + 1);
+    FOPIS_HIP(hipMemsetAsync(b.ocnt.p, 0, (nt + 1) * sizeof(int32_t), st));
+    int64_t tot[2];
+    FOPIS_HIP(hipMemcpyAsync(&tot[0], b.start.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    FOPIS_HIP(hipMemcpyAsync(&tot[1], b.ostart.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    FOPIS_HIP(hipStreamSynchronize(st));
+    if ((size_t)tot[0] != b.reg.size()) b.reg.alloc(std::max<int64_t>(tot[0], 1));
+    if ((size_t)tot[1] != b.out.size()) {
+      b.out.alloc(std::max<int64_t>(tot[1], 1));
+      b.odir.alloc(std::max<int64_t>(tot[1], 1));
+    }
+    b.total_cap = tot[0];
+    const int64_t want_stray = std::max<int64_t>(1 << 16, tot[0] / 32);
+    if (b.stray_cap < want_stray) {
+      b.stray.alloc(want_stray);
+      b.stray_cap = want_stray;
+    }
+  }
+  if (n > 0) {
+    SoaPtr<R> dst = ptrs(b.reg);
+    k_fill_regions<R><<<div_up(n, 256), 256, 0, st>>>(dst, b.start.p, ptrs(flat), sc.keys.p, sc.perm.p, n, shift,
+                                                      sc.first.p, (uint32_t)nt);
+  }
+  FOPIS_HIP(hipGetLastError());
+}
+
+// Copies every live particle of species s into `flat` (tile order, then the
+// strays) and returns the count.
+template <class R>
+int64_t gather_all(Ctx& c, int s, Soa<R>& flat, bool with_strays) {
+  auto& b = bins<R>(c)[s];
+  Scratch& sc = scratch(c);
+  hipStream_t st = c.stream;
+  const int nt = c.ntiles;
+  if (!b.cnt.p) return 0;
+  sc.cnt.alloc(std::max<size_t>(sc.cnt.n, nt + 1));
+  sc.first.alloc(std::max<size_t>(sc.first.n, nt + 1));
+  k_u32_from_i32<<<div_up(nt, 256), 256, 0, st>>>(b.cnt.p, sc.cnt.p, nt);
+  FOPIS_HIP(hipMemsetAsync(sc.cnt.p + nt, 0, sizeof(uint32_t), st));
+  exclusive_scan_u32(st, sc.cnt.p, nt + 1, sc.first.p, sc.tmp);
+  int64_t nreg = 0;
+  int32_t stat[4] = {0, 0, 0, 0};
+  FOPIS_HIP(hipMemcpyAsync(&nreg, sc.first.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  FOPIS_HIP(hipMemcpyAsync(stat, c.status.p + 4 * s, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  FOPIS_HIP(hipStreamSynchronize(st));
+  if (stat[1] > 0) throw Error(FOPIS_E_NOMEM, "stray buffer overflow: particles were lost");
+  const int64_t nstray = with_strays ? std::min<int64_t>(stat[0], b.stray_cap) : 0;
+  const int64_t n = nreg + nstray;
+  if ((int64_t)flat.size() < n) flat.alloc(n);
+  if (nreg > 0) k_gather_regions<R><<<nt, kBlock, 0, st>>>(ptrs(flat), sc.first.p, ptrs(b.reg), b.start.p, b.cnt.p);
+  if (nstray > 0) k_copy_soa<R><<<div_up(nstray, 256), 256, 0, st>>>(ptrs(flat), nreg, ptrs(b.stray), nstray);
+  FOPIS_HIP(hipGetLastError());
+  return n;
+}
+
+template <class R>
+void rebin(Ctx& c, int s) {
+  Soa<R> flat;
+  const int64_t n = gather_all<R>(c, s, flat, true);
+  FOPIS_HIP(hipMemsetAsync(c.status.p + 4 * s, 0, 4 * sizeof(int32_t), c.stream));
+  bin_flat<R>(c, s, flat, n, false);
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Uploads the host-staged new particles and sorts them by tile.
+template <class R>
+void prepare_inbox(Ctx& c, int s, bool consume) {
+  auto& b = bins<R>(c)[s];
+  const int64_t n = (int64_t)b.host_inbox[0].size();
+  b.inbox_n = n;
+  if (n == 0) return;
+  hipStream_t st = c.stream;
+  Scratch& sc = scratch(c);
+  const int nt = c.ntiles;
+  Soa<double> raw;
+  raw.alloc(n);
+  for (int q = 0; q < 6; ++q)
+    FOPIS_HIP(hipMemcpyAsync(raw.c[q].p, b.host_inbox[q].data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+  TileArgs<double> a{};
+  for (int q = 0; q < 3; ++q) {
+    a.N[q] = c.Nc[q];
+    a.T[q] = c.nt[q];
+  }
+  a.xmax[0] = c.nx - 1;
+  a.xmax[1] = c.ny - 1;
+  a.xmax[2] = c.nz - 1;
+  sc.keys.alloc(std::max<size_t>(sc.keys.n, n));
+  sc.perm.alloc(std::max<size_t>(sc.perm.n, n));
+  int bits = 1;
+  while (((int64_t)1 << bits) < (int64_t)nt + 1) ++bits;
+  k_keys_tile<double><<<div_up(n, 256), 256, 0, st>>>(ptrs(raw), n, a, sc.keys.p, 0, (uint32_t)nt);
+  sc.sorter.sort(st, sc.keys.p, sc.perm.p, n, bits, true);
+  sc.cnt.alloc(std::max<size_t>(sc.cnt.n, nt + 1));
+  sc.first.alloc(std::max<size_t>(sc.first.n, nt + 1));
+  FOPIS_HIP(hipMemsetAsync(sc.cnt.p, 0, (nt + 1) * sizeof(uint32_t), st));
+  k_count_tiles<<<div_up(n, 256), 256, 0, st>>>(sc.keys.p, n, 0, (uint32_t)nt, sc.cnt.p);
+  exclusive_scan_u32(st, sc.cnt.p, nt + 1, sc.first.p, sc.tmp);
+  b.ibeg.alloc(nt + 1);
+  b.inbox.alloc(n);
+  // inbox in tile order (invalid ones sort past ibeg[nt] and are not copied)
+  k_fill_regions<double><<<div_up(n, 256), 256, 0, st>>>(ptrs(b.inbox), sc.first.p, ptrs(raw), sc.keys.p,
+                                                         sc.perm.p, n, 0, sc.first.p, (uint32_t)nt);
+  std::vector<int64_t> first(nt + 1);
+  FOPIS_HIP(hipMemcpyAsync(first.data(), sc.first.p, (nt + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  FOPIS_HIP(hipStreamSynchronize(st));
+  std::vector<int32_t> ib(nt + 1);
+  for (int q = 0; q <= nt; ++q) ib[q] = (int32_t)first[q];
+  FOPIS_HIP(hipMemcpyAsync(b.ibeg.p, ib.data(), (nt + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  FOPIS_HIP(hipStreamSynchronize(st));
+  if (consume)
+    for (auto& v : b.host_inbox) {
+      v.clear();
+      v.shrink_to_fit();
+    }
+}
+
+template <class R, int MODE>
+void launch_tiles(Ctx& c, int s) {
+  TileArgs<R> a = tile_args<R>(c, s);
+  k_tile<R, MODE><<<c.ntiles, kBlock, 0, c.stream>>>(a);
+  FOPIS_HIP(hipGetLastError());
+}
+
+template <class R>
+void after_bin(Ctx& c) {
+  // strays -> full re-bin of that species
+  std::vector<int32_t> stat(4 * c.S);
+  FOPIS_HIP(hipMemcpyAsync(stat.data(), c.status.p, 4 * c.S * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  for (int s = 0; s < c.S; ++s) {
+    if (stat[4 * s + 1] > 0) throw Error(FOPIS_E_NOMEM, "stray buffer overflow: particles were lost");
+    if (stat[4 * s + 0] > 0) rebin<R>(c, s);
+  }
+}
+
+template <class R>
+void deposit_bin(Ctx& c, bool push) {
+  hipStream_t st = c.stream;
+  for (int s = 0; s < c.S; ++s) prepare_inbox<R>(c, s, true);
+  FOPIS_HIP(hipMemsetAsync(c.rho.p, 0, c.rho.n * sizeof(double), st));
+  FOPIS_HIP(hipMemsetAsync(c.density.p, 0, c.density.n * sizeof(double), st));
+  FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), st));
+  for (int o = 0; o < c.nobj; ++o)
+    if (c.obj[o].absorbed.p) FOPIS_HIP(hipMemsetAsync(c.obj[o].absorbed.p, 0, c.S * sizeof(unsigned long long), st));
+  {
+    Phase ph(c, "push_deposit");
+    for (int s = 0; s < c.S; ++s) {
+      if (push)
+        launch_tiles<R, M_VEL | M_POS | M_DEP | M_BIN>(c, s);
+      else
+        launch_tiles<R, M_DEP | M_BIN>(c, s);
+    }
+  }
+  {
+    Phase ph(c, "bin");
+    for (int s = 0; s < c.S; ++s) {
+      TileArgs<R> a = tile_args<R>(c, s);
+      k_arrive<R><<<c.ntiles, kBlock, 0, st>>>(a);
+    }
+    FOPIS_HIP(hipGetLastError());
+  }
+  for (int s = 0; s < c.S; ++s) bins<R>(c)[s].inbox_n = 0;
+  after_bin<R>(c);
+  fields_reduce_windows(c, true, true);
+}
+
+}  // namespace
+
+void particles_init(Ctx& c) {
+  if (c.fp32)
+    c.sp32.resize(c.S);
+  else
+    c.sp64.resize(c.S);
+  // rho_part holds S*ntiles windows of R (FP32 windows use half the bytes)
+  c.rho_part.alloc((size_t)c.S * c.ntiles * (kRW * kRW * kRW));
+  c.dens_part.alloc((size_t)c.S * c.ntiles * (kDW * kDW * kDW));
+  c.status.alloc(4 * FOPIS_MAX_SPECIES);
+  FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), c.stream));
+  for (int s = 0; s < c.S; ++s) {
+    if (c.fp32)
+      bin_flat<float>(c, s, c.sp32[s].reg, 0, false);
+    else
+      bin_flat<double>(c, s, c.sp64[s].reg, 0, false);
+  }
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+}
+
+template <class R>
+static void set_impl(Ctx& c, int s, int64_t n, const double* const* h) {
+  hipStream_t st = c.stream;
+  Soa<R> flat;
+  if (n > 0) {
+    flat.alloc(n);
+    Soa<double> raw;
+    raw.alloc(n);
+    for (int q = 0; q < 6; ++q)
+      FOPIS_HIP(hipMemcpyAsync(raw.c[q].p, h[q], n * sizeof(double), hipMemcpyHostToDevice, st));
+    k_from_f64<R><<<div_up(n, 256), 256, 0, st>>>(ptrs(flat), ptrs(raw), n);
+    FOPIS_HIP(hipStreamSynchronize(st));
+  }
+  FOPIS_HIP(hipMemsetAsync(c.status.p + 4 * s, 0, 4 * sizeof(int32_t), st));
+  bin_flat<R>(c, s, flat, n, false);
+  FOPIS_HIP(hipStreamSynchronize(st));
+}
+
+void particles_set(Ctx& c, int s, int64_t n, const double* const* h) {
+  if (c.fp32)
+    set_impl<float>(c, s, n, h);
+  else
+    set_impl<double>(c, s, n, h);
+}
+
+template <class R>
+static int64_t count_impl(Ctx& c, int s) {
+  auto& b = bins<R>(c)[s];
+  std::vector<int32_t> cnt(c.ntiles);
+  FOPIS_HIP(hipMemcpyAsync(cnt.data(), b.cnt.p, c.ntiles * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  int64_t n = 0;
+  for (int v : cnt) n += v;
+  return n + (int64_t)b.host_inbox[0].size();
+}
+
+int64_t particles_count(Ctx& c, int s) { return c.fp32 ? count_impl<float>(c, s) : count_impl<double>(c, s); }
+
+template <class R>
+static void get_impl(Ctx& c, int s, int64_t cap, double* const* h, int64_t* nout) {
+  Soa<R> flat;
+  const int64_t n = gather_all<R>(c, s, flat, false);
+  *nout = n;
+  if (n > cap) throw Error(FOPIS_E_ARG, "output buffer too small: need " + std::to_string(n));
+  if (n == 0) return;
+  Soa<double> out;
+  out.alloc(n);
+  k_to_f64<R><<<div_up(n, 256), 256, 0, c.stream>>>(ptrs(out), ptrs(flat), n);
+  for (int q = 0; q < 6; ++q)
+    FOPIS_HIP(hipMemcpyAsync(h[q], out.c[q].p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+}
+
+void particles_get(Ctx& c, int s, int64_t cap, double* const* h, int64_t* n) {
+  if (c.fp32)
+    get_impl<float>(c, s, cap, h, n);
+  else
+    get_impl<double>(c, s, cap, h, n);
+}
+
+void particles_stage(Ctx& c, int s, int64_t n, const double* const* h) {
+  auto& hb = c.fp32 ? c.sp32[s].host_inbox : c.sp64[s].host_inbox;
+  for (int q = 0; q < 6; ++q) hb[q].insert(hb[q].end(), h[q], h[q] + n);
+}
+
+void particles_velocity(Ctx& c) {
+  Phase ph(c, "velocity");
+  for (int s = 0; s < c.S; ++s) {
+    if (c.fp32)
+      launch_tiles<float, M_VEL>(c, s);
+    else
+      launch_tiles<double, M_VEL>(c, s);
+  }
+}
+
+void particles_position(Ctx& c) {
+  Phase ph(c, "position");
+  for (int s = 0; s < c.S; ++s) {
+    if (c.fp32)
+      launch_tiles<float, M_POS>(c, s);
+    else
+      launch_tiles<double, M_POS>(c, s);
+  }
+}
+
+void particles_density_only(Ctx& c) {
+  for (int s = 0; s < c.S; ++s) {
+    if (c.fp32)
+      prepare_inbox<float>(c, s, false);
+    else
+      prepare_inbox<double>(c, s, false);
+  }
+  FOPIS_HIP(hipMemsetAsync(c.density.p, 0, c.density.n * sizeof(double), c.stream));
+  for (int s = 0; s < c.S; ++s) {
+    if (c.fp32)
+      launch_tiles<float, M_DENS>(c, s);
+    else
+      launch_tiles<double, M_DENS>(c, s);
+  }
+  for (int s = 0; s < c.S; ++s) {
+    if (c.fp32)
+      c.sp32[s].inbox_n = 0;
+    else
+      c.sp64[s].inbox_n = 0;
+  }
+  fields_reduce_windows(c, false, true);
+}
+
+void particles_deposit_bin(Ctx& c, bool push) {
+  if (c.fp32)
+    deposit_bin<float>(c, push);
+  else
+    deposit_bin<double>(c, push);
+}
+
+template <class R>
+static void sort_cell_impl(Ctx& c) {
+  for (int s = 0; s < c.S; ++s) {
+    Soa<R> flat;
+    rebin<R>(c, s);  // fold strays in first so every tile holds exactly its particles
+    const int64_t n = gather_all<R>(c, s, flat, false);
+    bin_flat<R>(c, s, flat, n, true);
+    FOPIS_HIP(hipStreamSynchronize(c.stream));
+  }
+}
+
+void particles_sort_by_cell(Ctx& c) {
+  if (c.fp32)
+    sort_cell_impl<float>(c);
+  else
+    sort_cell_impl<double>(c);
+}
+
+template <class R>
+static void keys_impl(Ctx& c, int s, int64_t cap, int32_t* keys, int64_t* nout) {
+  Soa<R> flat;
+  const int64_t n = gather_all<R>(c, s, flat, false);
+  *nout = n;
+  if (n > cap) throw Error(FOPIS_E_ARG, "output buffer too small");
+  if (n == 0) return;
+  DBuf<int32_t> k;
+  k.alloc(n);
+  k_cell_keys<R><<<div_up(n, 256), 256, 0, c.stream>>>(ptrs(flat), n, c.ny + 1, c.nz + 1, k.p);
+  FOPIS_HIP(hipMemcpyAsync(keys, k.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+}
+
+void particles_cell_keys(Ctx& c, int s, int64_t cap, int32_t* keys, int64_t* n) {
+  if (c.fp32)
+    keys_impl<float>(c, s, cap, keys, n);
+  else
+    keys_impl<double>(c, s, cap, keys, n);
+}
+
+template <class R>
+static double energy_impl(Ctx& c) {
+  double res = 0.0;
+  c.red.alloc(std::max<size_t>(c.red.n, 1025));
+  for (int s = 0; s < c.S; ++s) {
+    Soa<R> flat;
+    const int64_t n = gather_all<R>(c, s, flat, false);
+    double e = 0.0;
+    if (n > 0) {
+      const int nb = 1024;
+      k_v2<R><<<nb, kBlock, 0, c.stream>>>(ptrs(flat), n, c.red.p);
+      k_sum_serial<<<1, 1, 0, c.stream>>>(c.red.p, nb, c.red.p + 1024);
+      FOPIS_HIP(hipMemcpyAsync(&e, c.red.p + 1024, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+      FOPIS_HIP(hipStreamSynchronize(c.stream));
+    }
+    const fopis_species_desc& sd = c.d.species[s];
+    res += 0.5 * sd.mass * e * sd.size;
+  }
+  return res;
+}
+
+double particles_energy(Ctx& c) { return c.fp32 ? energy_impl<float>(c) : energy_impl<double>(c); }
+
+}  // namespace fopis

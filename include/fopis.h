@@ -1,0 +1,196 @@
+/*
+ * fopis.h — C ABI of the MI355X electrostatic PIC step (libfopis_hip.so).
+ *
+ * The reference (hsimyu/fopis) has no foreign-function interface: its hot path
+ * sits behind the C++ class surface of Grid/RootGrid/Field/Spacecraft. Each
+ * entry point below replaces one of those methods and says which (file:line in
+ * the reference). A C++ host facade (fopis_amd/csrc/host_grid.hpp, class
+ * fopis::GpuRootGrid) keeps the reference's method names on top of this ABI.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative FOPIS_E_* code on
+ *    failure; fopis_last_error() then holds a message (thread-local). Nothing
+ *    throws across the ABI.
+ *  - Plain pointers and sizes only; all host buffers stay owned by the caller
+ *    and are copied. The library owns every device buffer.
+ *  - Arrays of grid quantities use the reference's layout: C order, glue layer
+ *    included, node arrays (nx+2)*(ny+2)*(nz+2) doubles (include/field.hpp:5-36,
+ *    src/Grid/Grid.cpp:148-196).
+ *  - All quantities are normalised as in the reference (dx = dt = 1, masses in
+ *    electron masses, charges in elementary charges; include/normalizer.hpp).
+ *  - One host thread drives one context; work is ordered on the context's
+ *    HIP stream; calls that return data synchronise that stream.
+ */
+#ifndef FOPIS_H
+#define FOPIS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FOPIS_MAX_SPECIES 8
+#define FOPIS_MAX_OBJECTS 4
+
+enum fopis_status {
+  FOPIS_OK = 0,
+  FOPIS_E_ARG = -1,      /* bad argument (std::invalid_argument in the reference) */
+  FOPIS_E_STATE = -2,    /* call out of order (std::logic_error)                  */
+  FOPIS_E_DEVICE = -3,   /* HIP runtime error / no device / library not built     */
+  FOPIS_E_NOMEM = -4,    /* device allocation failed                              */
+  FOPIS_E_NOCONV = -5    /* Poisson solve hit its cycle limit before the tolerance */
+};
+
+enum fopis_precision { FOPIS_FP64 = 0, FOPIS_FP32 = 1 };
+
+/* Field selectors for fopis_get_field / fopis_set_field (include/field.hpp:57-99). */
+enum fopis_field {
+  FOPIS_PHI = 0,       /* (nx+2)^3                                   */
+  FOPIS_RHO = 1,       /* (nx+2)^3; index 0 = total, s+1 = species s */
+  FOPIS_DENSITY = 2,   /* (nx+1)^3 per species (cell-centred)        */
+  FOPIS_EX = 3,        /* (nx+1)(ny+2)(nz+2)                         */
+  FOPIS_EY = 4,        /* (nx+2)(ny+1)(nz+2)                         */
+  FOPIS_EZ = 5,        /* (nx+2)(ny+2)(nz+1)                         */
+  FOPIS_EXREF = 6,     /* (nx+2)^3                                   */
+  FOPIS_EYREF = 7,
+  FOPIS_EZREF = 8
+};
+
+typedef struct fopis_ctx fopis_ctx;
+
+typedef struct fopis_species_desc {
+  double charge; /* ParticleType::charge, elementary charges (particle_type.hpp:89)   */
+  double mass;   /* ParticleType::mass, electron masses (particle_type.hpp:88)        */
+  double size;   /* ParticleType::size, real particles per superparticle (:96)        */
+} fopis_species_desc;
+
+typedef struct fopis_desc {
+  int32_t nx, ny, nz;        /* RootGrid nx/ny/nz: node counts incl. the two face nodes */
+  char boundary[8];          /* "DDDDDD" (Environment::boundary); D and N supported     */
+  int32_t nspecies;
+  fopis_species_desc species[FOPIS_MAX_SPECIES];
+  double static_b[3];        /* normalised static B (Initialize.cpp:306-327)            */
+  double eps0;               /* Normalizer::eps0                                        */
+  int32_t precision;         /* particles, push and deposit: FOPIS_FP64 | FOPIS_FP32    */
+  int32_t mg_precision;      /* V-cycle smoothing: FOPIS_FP64 | FOPIS_FP32              */
+  double mg_omega;           /* red-black SOR relaxation in the V-cycle (0 -> 1.5)      */
+  int32_t mg_pre, mg_post;   /* smoothing sweeps (0 -> 2, 2)                            */
+  int32_t mg_max_cycles;     /* defect-correction cycle limit (0 -> 100)                */
+  double poisson_tol;        /* ||b - A phi||_2 / ||b||_2 target (0 -> 1e-8)            */
+  double capacity_slack;     /* per-tile capacity factor (0 -> 1.25)                    */
+} fopis_desc;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+
+/* RootGrid::RootGrid field allocation (RootGrid.cpp:7-44, Grid.cpp:148-196). */
+int fopis_create(const fopis_desc* desc, fopis_ctx** out);
+int fopis_destroy(fopis_ctx* ctx);
+const char* fopis_last_error(void);
+/* Library version string and the device it runs on. */
+int fopis_device_info(char* buf, int buflen);
+
+/* ---- particles -------------------------------------------------------------- */
+
+/* Replaces the initial load's push_back into Grid::particles[s]
+ * (RootGrid.cpp:57-79): n particles of species s, FP64 SoA, replace the
+ * species' current set. Particles are tile-binned on the device. */
+int fopis_set_particles(fopis_ctx* ctx, int s, int64_t n, const double* x, const double* y, const double* z,
+                        const double* vx, const double* vy, const double* vz);
+/* Number of live particles of species s (Grid::getValidParticleNumber,
+ * GridParticleTreatment.cpp:120-128). */
+int fopis_num_particles(fopis_ctx* ctx, int s, int64_t* n);
+/* Copies species s out in device order (tile-major). cap = buffer length. */
+int fopis_get_particles(fopis_ctx* ctx, int s, int64_t cap, double* x, double* y, double* z, double* vx,
+                        double* vy, double* vz, int64_t* n);
+/* Stages new particles of species s (injected at the boundary or emitted by
+ * an object) to join at the next deposit: replaces particles[pid].push_back in
+ * RootGrid::injectParticlesFromBoundary (RootGridParticle.cpp:46-127) and
+ * Spacecraft::emitParticles (Spacecraft.cpp:887-897). */
+int fopis_add_particles(fopis_ctx* ctx, int s, int64_t n, const double* x, const double* y, const double* z,
+                        const double* vx, const double* vy, const double* vz);
+/* Stable counting sort of every species by the reference cell key
+ * ((i*(ny+1)+j)*(nz+1)+k with i = floor(x)+1, position.hpp:59-69); the order
+ * is then what std::stable_sort by that key gives on the device order. */
+int fopis_sort_by_cell(fopis_ctx* ctx);
+/* Reference cell keys of species s in device order. */
+int fopis_get_cell_keys(fopis_ctx* ctx, int s, int64_t cap, int32_t* keys, int64_t* n);
+
+/* ---- the step, method by method -------------------------------------------- */
+
+/* Grid::updateParticleVelocityES (GridParticleTreatmentES.cpp:13-88). */
+int fopis_update_particle_velocity_es(fopis_ctx* ctx);
+/* RootGrid::updateParticlePositionES (GridParticleTreatmentES.cpp:90-181,
+ * grid.hpp:309-395): x += v; a particle leaving a physical face is invalid. */
+int fopis_update_particle_position_es(fopis_ctx* ctx);
+/* RootGrid::updateDensity (RootGridField.cpp:1129-1150). */
+int fopis_update_density(fopis_ctx* ctx);
+/* RootGrid::updateRho (RootGridField.cpp:1153-1252): conductor absorption,
+ * CIC deposit, rho[0] = sum, and with objects the first solve plus the
+ * capacitance redistribution (Spacecraft.cpp:708-758). Also drops invalid
+ * particles and re-bins the rest (GridParticleTreatment.cpp:93-102). */
+int fopis_update_rho(fopis_ctx* ctx);
+/* RootGrid::solvePoisson (RootGridField.cpp:41-103): FP64 defect correction
+ * around V-cycles to poisson_tol, warm-started from the current phi. Either
+ * out pointer may be NULL. */
+int fopis_solve_poisson(fopis_ctx* ctx, int32_t* cycles, double* relres);
+/* RootGrid::updateEfield + updateReferenceEfield (RootGridField.cpp:456-582). */
+int fopis_update_efield(fopis_ctx* ctx);
+/* RootGrid::mainLoopES (RootGrid.cpp:82-143) on the device, fused: one
+ * kernel per species pushes, moves, absorbs, counts, deposits and re-bins;
+ * then rho reduction, conductor correction, Poisson and E. Particles staged
+ * by fopis_add_particles join this step's deposit. */
+int fopis_main_loop_es(fopis_ctx* ctx);
+/* Grid::initialize (Grid.cpp:23-36): updateRho, solvePoisson, updateEfield. */
+int fopis_initialize(fopis_ctx* ctx);
+
+/* ---- fields ---------------------------------------------------------------- */
+
+int fopis_get_field(fopis_ctx* ctx, int which, int s, double* out);
+int fopis_set_field(fopis_ctx* ctx, int which, int s, const double* in);
+/* Grid::getParticleEnergy (GridParticleTreatment.cpp:104-118) and
+ * Field::getEfieldEnergy (Field.cpp:194-213), FP64 reductions on the device.
+ * out[0] = particle energy, out[1] = field energy. */
+int fopis_energy(fopis_ctx* ctx, double* out);
+
+/* ---- conductors (Spacecraft) ------------------------------------------------- */
+
+/* Spacecraft::construct for a perfect conductor (Spacecraft.cpp:16-152):
+ * cell_mask is the reference object_cell_map, (nx+1)(ny+1)(nz+1) bytes over
+ * glue-relative cells; cmat_ijk holds the glue-relative node of each of the
+ * ncmat capacitance-matrix nodes. fixed != 0 holds the potential at
+ * fixed_potential (normalised). */
+int fopis_set_object(fopis_ctx* ctx, int obj, const uint8_t* cell_mask, int32_t ncmat, const int32_t* cmat_ijk,
+                     int32_t fixed, double fixed_potential);
+/* RootGrid::initializeObjectsCmatrix (RootGridObject.cpp:77-148): one unit
+ * charge solve per surface node on the device, then C = B^-1 in FP64. */
+int fopis_build_cmatrix(fopis_ctx* ctx, int obj);
+/* Read or load C (element (col,row) at col*ncmat+row, Cmatrix.hpp:17-20);
+ * loading replaces IO::loadCmatrixData (IO.cpp:59-120). */
+int fopis_get_cmatrix(fopis_ctx* ctx, int obj, double* out);
+int fopis_set_cmatrix(fopis_ctx* ctx, int obj, const double* in);
+/* Adds host-side surface charge (photoemission, Spacecraft.cpp:945-970) to
+ * the object's temporary charge map, S*ncmat doubles, species-major. */
+int fopis_add_object_charge(fopis_ctx* ctx, int obj, const double* dq, const double* dcurrent);
+/* State after the last redistribution: potential, total charge, per-species
+ * current (S), charge_map (S*ncmat, may be NULL). */
+int fopis_get_object_state(fopis_ctx* ctx, int obj, double* potential, double* total_charge, double* current,
+                           double* charge_map);
+/* Sets charge_map (S*ncmat), as initializeChargeMapOffset does (Spacecraft.cpp:160-183). */
+int fopis_set_object_charge_map(fopis_ctx* ctx, int obj, const double* charge_map);
+
+/* ---- measurement ------------------------------------------------------------- */
+
+/* Average device time (ms) of the named phase over the calls since the last
+ * reset, measured with HIP events on the context's stream: "push_deposit",
+ * "poisson", "efield", "bin". n = number of samples. */
+int fopis_phase_time(fopis_ctx* ctx, const char* phase, double* avg_ms, int64_t* n);
+int fopis_reset_timers(fopis_ctx* ctx);
+/* Blocks until the context's stream is idle. */
+int fopis_synchronize(fopis_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FOPIS_H */
