@@ -1,0 +1,172 @@
+"""Benchmark of the MI355X ES PIC step (one GPU).
+
+Default workload (configs C2): 128^3 cells, 64 particles per cell for each of
+electrons and ions (2 x 133M superparticles, FP64), a floating 16^3-cell cube
+conductor handled by the capacitance matrix, upstream x-face injection,
+multigrid Poisson to relative residual 1e-8 (two solves per step because of
+the conductor). Particles are drawn on the device (Philox Maxwellian,
+synthetic); injection is drawn on the host from the reference's streams.
+
+A step is one RootGrid::mainLoopES. `value` is particles/s of the push+deposit
+part of the step (gather, Boris, move, absorb, count, CIC deposit, re-binning
+and the window reduction into rho), timed with HIP events on the library's
+stream inside the timed steps; ms_per_step is the whole step by wall clock.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particles/s (push+CIC deposit); Poisson ms to rel-res 1e-8, learned vs plain"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c2": {"json": "c2.json", "bytes_per_particle": {0: 96, 1: 48},
+           "name": "C2: 128^3 cells, 64 ppc x 2 species, floating 16^3 cube, x-face injection, FP64"},
+    "c3": {"json": "c3.json", "bytes_per_particle": {0: 96, 1: 48},
+           "name": "C3: 256^3 cells, 64 ppc x 2 species, floating 32^3 cube, B0=0.2 z, FP32 push/deposit"},
+    "c1": {"json": "c1.json", "bytes_per_particle": {0: 96, 1: 48},
+           "name": "C1: 32^3 cells, 20 ppc x 2 species, no object, FP64"},
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg_name: str, threads: int, ppc: int = 4, steps: int = 2) -> dict:
+    """The oracle (CPU restatement of the reference step, AoS FP64, OpenMP over
+    particles) timed on push + move + deposit at the workload's grid with a
+    reduced particle count."""
+    from oracle import refcpu
+    from tests.cases import load_config, oracle_params
+
+    refcpu.set_threads(threads)
+    cfg = load_config(cfg_name)
+    g = refcpu.Grid(oracle_params(cfg, ppc=ppc))
+    g.call("rc_load_initial_particles")
+    n = sum(g.num_particles(s) for s in range(g.S))
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        g.call("rc_update_velocity")
+        g.call("rc_update_position")
+        g.call("rc_deposit_only")
+        ts.append(time.perf_counter() - t0)
+    best = min(ts)
+    nx = cfg["Environment"]["nx"]
+    return {"value": n / best, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": f"oracle push+move+deposit (reference algorithm, FP64 AoS) on the {nx - 1}^3 grid with the "
+                      f"conductor, {ppc} ppc per species ({n} particles), best of {steps} steps of {best:.2f} s"}
+
+
+def pmc_traffic(workload: str):
+    p = os.path.join(ROOT, "profiles", f"tile_kernel_{workload}_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--ppc", type=int, default=0, help="override particles per cell (0 = config)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cmatrix-cache", default="")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from fopis_amd import lib as F
+    from fopis_amd.host import Host
+
+    wl = WORKLOADS[args.workload]
+    log(F.device_info())
+    t0 = time.perf_counter()
+    h = Host(os.path.join(ROOT, "fopis_amd", "configs", wl["json"]), device_loader=True, device_seed=20251202,
+             ppc_override=args.ppc, build_cmatrix=True, initialize=True, cmatrix_cache=args.cmatrix_cache)
+    ctx = h.ctx
+    nparts = sum(ctx.num_particles(s) for s in range(h.S))
+    log(f"setup {time.perf_counter() - t0:.1f} s, {nparts} particles")
+    for i in range(args.warmup):
+        h.main_loop_es()
+    ctx.synchronize()
+    log("warmup done")
+    ctx.reset_timers()
+    counts = []
+    gen_ms = 0.0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        h.main_loop_es()
+        gen_ms += h.generate_ms()
+        if (i + 1) % max(1, args.steps // 5) == 0:
+            log(f"step {i + 1}/{args.steps}")
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    counts = [ctx.num_particles(s) for s in range(h.S)]
+    n_end = sum(counts)
+    part_ms, n_part = ctx.phase_time("particles")
+    tile_ms, n_tile = ctx.phase_time("tile")
+    bin_ms, _ = ctx.phase_time("bin")
+    red_ms, _ = ctx.phase_time("reduce")
+    poi_ms, n_poi = ctx.phase_time("poisson")
+    ef_ms, _ = ctx.phase_time("efield")
+    n_avg = 0.5 * (nparts + n_end)
+    value = n_avg / (part_ms * 1e-3)
+    bpp = wl["bytes_per_particle"][h.desc.precision]
+    achieved = bpp * n_avg / (tile_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if h.desc.precision == 0 else "f32",
+        "data": "synthetic (device Philox Maxwellian load; host mt19937 injection)",
+        "config": {"workload": wl["name"], "particles": int(n_avg), "ppc": args.ppc or None,
+                   "poisson_tol": 1e-8, "mg": "V(2,2) red-black SOR omega=1.5, FP64 defect correction"},
+        "push_deposit_ms": part_ms,
+        "tile_kernel_ms": tile_ms,
+        "bin_ms": bin_ms,
+        "reduce_ms": red_ms,
+        "poisson_ms": poi_ms,
+        "poisson_solves_per_step": n_poi / max(1, n_part),
+        "efield_ms": ef_ms,
+        "host_generate_ms_per_step": gen_ms / args.steps,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_tile (gather+Boris+move+absorb+count+deposit+bin), all species per step",
+                     "algorithmic_bytes_per_particle": bpp},
+    }
+    tr = pmc_traffic(args.workload)
+    if tr:
+        line["roofline"]["traffic"] = tr.get("bytes_per_step")
+    h.close()
+    if not args.no_cpu_baseline:
+        try:
+            threads = min(16, os.cpu_count() or 1)
+            line["cpu_baseline"] = cpu_baseline(args.workload, threads)
+        except Exception as e:  # the oracle is optional on a box without it
+            line["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

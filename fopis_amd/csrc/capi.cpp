@@ -191,6 +191,16 @@ int fopis_set_particles(fopis_ctx* ctx, int s, int64_t n, const double* x, const
   });
 }
 
+int fopis_load_maxwellian(fopis_ctx* ctx, int s, int64_t n, uint64_t seed, const double* lo, const double* hi,
+                          const double* drift, double sigma) {
+  NEED_CTX
+  return guarded([&] {
+    if (s < 0 || s >= ctx->c.S || n < 0 || !lo || !hi || !drift || !(sigma >= 0.0))
+      throw Error(FOPIS_E_ARG, "bad argument");
+    particles_load_maxwellian(ctx->c, s, n, seed, lo, hi, drift, sigma);
+  });
+}
+
 int fopis_num_particles(fopis_ctx* ctx, int s, int64_t* n) {
   NEED_CTX
   return guarded([&] {

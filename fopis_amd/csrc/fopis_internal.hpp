@@ -199,6 +199,8 @@ struct Ctx {
 // ---- entry points implemented in the .hip units ----
 void particles_init(Ctx& c);
 void particles_set(Ctx& c, int s, int64_t n, const double* const* host6);
+void particles_load_maxwellian(Ctx& c, int s, int64_t n, uint64_t seed, const double* lo, const double* hi,
+                               const double* drift, double sigma);
 int64_t particles_count(Ctx& c, int s);
 void particles_get(Ctx& c, int s, int64_t cap, double* const* host6, int64_t* n);
 void particles_stage(Ctx& c, int s, int64_t n, const double* const* host6);

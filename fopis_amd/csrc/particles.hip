@@ -739,6 +739,69 @@ __global__ void k_sum_serial(const double* part, int n, double* out) {
   }
 }
 
+// ------------------------------------------------------- device generator
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers:
+// as easy as 1, 2, 3"): 10 rounds of two 32x32->64 multiplies and a Weyl key.
+struct U4 {
+  uint32_t v[4];
+};
+
+__device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.v[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.v[2];
+    U4 n;
+    n.v[0] = (uint32_t)(p1 >> 32) ^ c.v[1] ^ k0;
+    n.v[1] = (uint32_t)p1;
+    n.v[2] = (uint32_t)(p0 >> 32) ^ c.v[3] ^ k1;
+    n.v[3] = (uint32_t)p0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// 53-bit uniform in [0, 1) from two 32-bit words
+__device__ __forceinline__ double u01(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+template <class R>
+__global__ void k_load_maxwellian(SoaPtr<R> f, int64_t n, uint64_t seed, double3 lo, double3 hi, double3 drift,
+                                  double sigma, const uint8_t* mask, int cy1, int cz1) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  U4 c{{(uint32_t)i, (uint32_t)(i >> 32), 0u, 0u}};
+  const U4 a = philox(c, k0, k1);
+  c.v[2] = 1u;
+  const U4 b = philox(c, k0, k1);
+  c.v[2] = 2u;
+  const U4 d = philox(c, k0, k1);
+  const double x = lo.x + (hi.x - lo.x) * u01(a.v[0], a.v[1]);
+  const double y = lo.y + (hi.y - lo.y) * u01(a.v[2], a.v[3]);
+  const double z = lo.z + (hi.z - lo.z) * u01(b.v[0], b.v[1]);
+  const double r1 = sqrt(-2.0 * log(1.0 - u01(b.v[2], b.v[3])));
+  const double t1 = 6.283185307179586 * u01(d.v[0], d.v[1]);
+  c.v[2] = 3u;
+  const U4 e = philox(c, k0, k1);
+  const double r2 = sqrt(-2.0 * log(1.0 - u01(d.v[2], d.v[3])));
+  const double t2 = 6.283185307179586 * u01(e.v[0], e.v[1]);
+  R px = (R)x, py = (R)y, pz = (R)z;
+  if (mask) {
+    const int ci = (int)floor(x) + 1, cj = (int)floor(y) + 1, ck = (int)floor(z) + 1;
+    if (mask[((size_t)ci * cy1 + cj) * cz1 + ck] != 0) px = (R)NAN;
+  }
+  f.c[0][i] = px;
+  f.c[1][i] = py;
+  f.c[2][i] = pz;
+  f.c[3][i] = (R)(drift.x + sigma * (r1 * cos(t1)));
+  f.c[4][i] = (R)(drift.y + sigma * (r1 * sin(t1)));
+  f.c[5][i] = (R)(drift.z + sigma * (r2 * cos(t2)));
+}
+
 // ------------------------------------------------------------------ host side
 
 namespace {
@@ -1003,6 +1066,7 @@ void after_bin(Ctx& c) {
 template <class R>
 void deposit_bin(Ctx& c, bool push) {
   hipStream_t st = c.stream;
+  Phase whole(c, "particles");
   for (int s = 0; s < c.S; ++s) prepare_inbox<R>(c, s, true);
   FOPIS_HIP(hipMemsetAsync(c.rho.p, 0, c.rho.n * sizeof(double), st));
   FOPIS_HIP(hipMemsetAsync(c.density.p, 0, c.density.n * sizeof(double), st));
@@ -1010,7 +1074,7 @@ void deposit_bin(Ctx& c, bool push) {
   for (int o = 0; o < c.nobj; ++o)
     if (c.obj[o].absorbed.p) FOPIS_HIP(hipMemsetAsync(c.obj[o].absorbed.p, 0, c.S * sizeof(unsigned long long), st));
   {
-    Phase ph(c, "push_deposit");
+    Phase ph(c, "tile");
     for (int s = 0; s < c.S; ++s) {
       if (push)
         launch_tiles<R, M_VEL | M_POS | M_DEP | M_BIN>(c, s);
@@ -1028,7 +1092,10 @@ void deposit_bin(Ctx& c, bool push) {
   }
   for (int s = 0; s < c.S; ++s) bins<R>(c)[s].inbox_n = 0;
   after_bin<R>(c);
-  fields_reduce_windows(c, true, true);
+  {
+    Phase ph(c, "reduce");
+    fields_reduce_windows(c, true, true);
+  }
 }
 
 }  // namespace
@@ -1068,6 +1135,34 @@ static void set_impl(Ctx& c, int s, int64_t n, const double* const* h) {
   FOPIS_HIP(hipMemsetAsync(c.status.p + 4 * s, 0, 4 * sizeof(int32_t), st));
   bin_flat<R>(c, s, flat, n, false);
   FOPIS_HIP(hipStreamSynchronize(st));
+}
+
+template <class R>
+static void load_impl(Ctx& c, int s, int64_t n, uint64_t seed, const double* lo, const double* hi,
+                      const double* drift, double sigma) {
+  Soa<R> flat;
+  if (n > 0) {
+    flat.alloc(n);
+    k_load_maxwellian<R><<<div_up(n, 256), 256, 0, c.stream>>>(
+        ptrs(flat), n, seed, make_double3(lo[0], lo[1], lo[2]), make_double3(hi[0], hi[1], hi[2]),
+        make_double3(drift[0], drift[1], drift[2]), sigma, c.have_mask ? c.mask.p : nullptr, c.cy - 1, c.cz - 1);
+    FOPIS_HIP(hipGetLastError());
+  }
+  FOPIS_HIP(hipMemsetAsync(c.status.p + 4 * s, 0, 4 * sizeof(int32_t), c.stream));
+  bin_flat<R>(c, s, flat, n, false);
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+}
+
+void particles_load_maxwellian(Ctx& c, int s, int64_t n, uint64_t seed, const double* lo, const double* hi,
+                               const double* drift, double sigma) {
+  for (int a = 0; a < 3; ++a) {
+    if (!(lo[a] >= 0.0) || !(hi[a] <= (double)(a == 0 ? c.nx : (a == 1 ? c.ny : c.nz)) - 1.0) || !(hi[a] > lo[a]))
+      throw Error(FOPIS_E_ARG, "load box must lie inside [0, n-1] with lo < hi");
+  }
+  if (c.fp32)
+    load_impl<float>(c, s, n, seed, lo, hi, drift, sigma);
+  else
+    load_impl<double>(c, s, n, seed, lo, hi, drift, sigma);
 }
 
 void particles_set(Ctx& c, int s, int64_t n, const double* const* h) {

@@ -97,6 +97,15 @@ int fopis_device_info(char* buf, int buflen);
  * species' current set. Particles are tile-binned on the device. */
 int fopis_set_particles(fopis_ctx* ctx, int s, int64_t n, const double* x, const double* y, const double* z,
                         const double* vx, const double* vy, const double* vz);
+/* Draws n particles of species s on the device and replaces the species'
+ * set: positions uniform in [lo, hi) per axis, velocities drift + sigma *
+ * N(0,1), from a counter-based generator (Philox4x32-10, Box-Muller) keyed by
+ * seed; particles born inside a conductor are dropped, as
+ * Spacecraft::removeInnerParticle does. A statistically equivalent stand-in
+ * for the reference's mt19937 initial load (RootGrid.cpp:57-79,
+ * AmbientParticleType.cpp:76-93) at sizes where drawing on the host is slow. */
+int fopis_load_maxwellian(fopis_ctx* ctx, int s, int64_t n, uint64_t seed, const double* lo, const double* hi,
+                          const double* drift, double sigma);
 /* Number of live particles of species s (Grid::getValidParticleNumber,
  * GridParticleTreatment.cpp:120-128). */
 int fopis_num_particles(fopis_ctx* ctx, int s, int64_t* n);
