@@ -362,11 +362,18 @@ struct Object {
   }
 };
 
+// One step's worth of host-drawn particles and emission charge.
+struct StepBatch {
+  std::vector<std::vector<Particle6>> particles;  // per species
+  std::vector<std::vector<double>> dq, dcur;       // per object: S*ncmat, S
+};
+
 // ---------------------------------------------------------- the facade
 class GpuRootGrid {
  public:
   GpuRootGrid(const std::string& path, const fopis_host_options& opt);
   ~GpuRootGrid() {
+    if (worker_.joinable()) worker_.join();
     if (ctx_) fopis_destroy(ctx_);
   }
   // RootGrid surface (RootGrid.cpp, RootGridField.cpp, RootGridParticle.cpp)
@@ -381,6 +388,10 @@ class GpuRootGrid {
   void injectParticlesFromBoundary();
   void emitParticlesFromObjects();
   std::vector<Particle6> generateInjection(int s);
+  void emitInto(StepBatch& b);
+  void injectInto(StepBatch& b);
+  void stageBatch(StepBatch& b);
+  StepBatch newBatch() const;
 
   fopis_ctx* ctx() { return ctx_; }
   const fopis_desc& desc() const { return d_; }
@@ -414,6 +425,9 @@ class GpuRootGrid {
   fopis_ctx* ctx_ = nullptr;
   int timestep_ = 1;
   double gen_ms_ = 0;
+  std::thread worker_;   // draws the next step's batch while the device runs this one
+  StepBatch next_;
+  bool have_next_ = false;
   std::string base_dir_;
 };
 
@@ -757,8 +771,41 @@ std::vector<Particle6> GpuRootGrid::generateInjection(int sid) {
   return out;
 }
 
+StepBatch GpuRootGrid::newBatch() const {
+  StepBatch b;
+  b.particles.resize(sp_.size());
+  b.dq.resize(objs_.size());
+  b.dcur.resize(objs_.size());
+  for (size_t o = 0; o < objs_.size(); ++o) {
+    b.dq[o].assign(sp_.size() * (size_t)objs_[o].ncmat, 0.0);
+    b.dcur[o].assign(sp_.size(), 0.0);
+  }
+  return b;
+}
+
+void GpuRootGrid::injectInto(StepBatch& b) {
+  for (size_t s = 0; s < sp_.size(); ++s) {
+    auto ps = generateInjection((int)s);
+    b.particles[s].insert(b.particles[s].end(), ps.begin(), ps.end());
+  }
+}
+
+void GpuRootGrid::stageBatch(StepBatch& b) {
+  for (size_t s = 0; s < sp_.size(); ++s) stage((int)s, b.particles[s]);
+  for (size_t o = 0; o < objs_.size(); ++o)
+    if (!objs_[o].emit_species.empty()) check(fopis_add_object_charge(ctx_, (int)o, b.dq[o].data(), b.dcur[o].data()));
+}
+
 void GpuRootGrid::injectParticlesFromBoundary() {
-  for (size_t s = 0; s < sp_.size(); ++s) stage((int)s, generateInjection((int)s));
+  StepBatch b = newBatch();
+  injectInto(b);
+  stageBatch(b);
+}
+
+void GpuRootGrid::emitParticlesFromObjects() {
+  StepBatch b = newBatch();
+  emitInto(b);
+  stageBatch(b);
 }
 
 // Spacecraft::emitParticles, photoelectron branch (Spacecraft.cpp:829-900),
@@ -766,7 +813,7 @@ void GpuRootGrid::injectParticlesFromBoundary() {
 // charge taken from the emitting face (Spacecraft.cpp:945-970). The emitted
 // particles then get this step's position update, as in the reference where
 // emission precedes updateParticlePosition.
-void GpuRootGrid::emitParticlesFromObjects() {
+void GpuRootGrid::emitInto(StepBatch& b) {
   for (int o = 0; o < (int)objs_.size(); ++o) {
     Object& ob = objs_[o];
     if (ob.emit_species.empty()) continue;
@@ -778,7 +825,8 @@ void GpuRootGrid::emitParticlesFromObjects() {
       }
     if (ax < 0) throw std::invalid_argument("At Spacecraft::emitParticles: Now 'shine_vector' must be [+-1,0,0] or [0,+-1,0] or [0,0,+-1].");
     const int ssign = shine_[ax] > 0.0 ? 1 : -1;
-    std::vector<double> dq((size_t)sp_.size() * ob.ncmat, 0.0), dcur(sp_.size(), 0.0);
+    std::vector<double>& dq = b.dq[o];
+    std::vector<double>& dcur = b.dcur[o];
     auto cell = [&](int i, int j, int k) { return ob.cell(nx_, ny_, nz_, i, j, k); };
     auto cmat = [&](int i, int j, int k) -> int {
       if (i < 0 || j < 0 || k < 0 || i > nx_ + 1 || j > ny_ + 1 || k > nz_ + 1) return -1;
@@ -794,7 +842,7 @@ void GpuRootGrid::emitParticlesFromObjects() {
       const double ev[3] = {-shine_[0], -shine_[1], -shine_[2]};
       const double vth = s.thermal_velocity(u_);
       std::uniform_real_distribution<> unit(0.0, 1.0);
-      std::vector<Particle6> born;
+      std::vector<Particle6>& born = b.particles[sid];
       for (int c = 0; c < ob.ncmat; ++c) {
         const int I = ob.geom.ijk[3 * c], J = ob.geom.ijk[3 * c + 1], K = ob.geom.ijk[3 * c + 2];
         // isXsurfacePoint(node, sign) for the lit face (Spacecraft.cpp:461-504)
@@ -896,22 +944,45 @@ void GpuRootGrid::emitParticlesFromObjects() {
           if (ok) born.push_back(np);
         }
       }
-      stage(sid, born);
     }
-    check(fopis_add_object_charge(ctx_, o, dq.data(), dcur.data()));
   }
 }
 
 // RootGrid::mainLoopES (RootGrid.cpp:82-143): the host draws this step's
 // emitted and injected particles (they need the reference's RNG streams),
 // stages them, and the device runs the fused step.
+//
+// The host streams do not depend on the plasma state, so the batch for step
+// n+1 is drawn on a worker thread while the device runs step n; the draws
+// happen in the reference's order either way.
 void GpuRootGrid::mainLoopES() {
   const auto t0 = std::chrono::steady_clock::now();
-  emitParticlesFromObjects();
-  injectParticlesFromBoundary();
+  StepBatch cur;
+  if (have_next_) {
+    worker_.join();
+    cur = std::move(next_);
+    have_next_ = false;
+  } else {
+    cur = newBatch();
+    emitInto(cur);
+    injectInto(cur);
+  }
+  stageBatch(cur);
   const auto t1 = std::chrono::steady_clock::now();
   gen_ms_ = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  check(fopis_main_loop_es(ctx_));
+  next_ = newBatch();
+  worker_ = std::thread([this] {
+    emitInto(next_);
+    injectInto(next_);
+  });
+  have_next_ = true;
+  try {
+    check(fopis_main_loop_es(ctx_));
+  } catch (...) {
+    worker_.join();
+    have_next_ = false;
+    throw;
+  }
   ++timestep_;
 }
 

@@ -58,6 +58,8 @@ struct TileArgs {
   const uint8_t* mask;
   int nobj;
   ObjArgs obj[FOPIS_MAX_OBJECTS];
+  double* cap_rec;  // captured particles: 8 doubles each (state, object)
+  int64_t cap_cap;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -84,20 +86,25 @@ __device__ __forceinline__ void set_xyz(DPos& p, double x, double y, double z) {
   p.dz2 = 1.0 - p.dz1;
 }
 
-template <class R>
+// Conductor geometry of one object for the capture path (copied by value so
+// the kernel arguments never need an address).
 struct Cond {
-  const TileArgs<R>& a;
-  int o;
+  int cx, cy, cz, o;
+  double q;
+  const uint8_t* mask;
+  const int32_t* node_cmat;
+  double* tmp;
+  unsigned long long* absorbed;
   __device__ bool valid_cell(int i, int j, int k) const {
-    return i >= 0 && i < a.cx - 1 && j >= 0 && j < a.cy - 1 && k >= 0 && k < a.cz - 1;
+    return i >= 0 && i < cx - 1 && j >= 0 && j < cy - 1 && k >= 0 && k < cz - 1;
   }
   __device__ bool cell(int i, int j, int k) const {
     if (!valid_cell(i, j, k)) return false;
-    return a.mask[((size_t)i * (a.cy - 1) + j) * (a.cz - 1) + k] == (uint8_t)(o + 1);
+    return mask[((size_t)i * (cy - 1) + j) * (cz - 1) + k] == (uint8_t)(o + 1);
   }
   __device__ int cmat(int i, int j, int k) const {
-    if (i < 0 || j < 0 || k < 0 || i >= a.cx || j >= a.cy || k >= a.cz) return -1;
-    return a.obj[o].node_cmat[((size_t)i * a.cy + j) * a.cz + k];
+    if (i < 0 || j < 0 || k < 0 || i >= cx || j >= cy || k >= cz) return -1;
+    return node_cmat[((size_t)i * cy + j) * cz + k];
   }
   // Spacecraft.cpp:461-504
   __device__ bool xpoint(const DPos& p, int sign) const {
@@ -125,26 +132,26 @@ struct Cond {
            cmat(p.i + 1, p.j + 1, p.k) >= 0;
   }
   __device__ void add(int c, double v) const {
-    if (c >= 0) atomicAdd(&a.obj[o].tmp[c], v);
+    if (c >= 0) atomicAdd(&tmp[c], v);
   }
   // Spacecraft.cpp:594-613
   __device__ void addx(const DPos& p) const {
-    add(cmat(p.i, p.j, p.k), a.q * p.dy2 * p.dz2);
-    add(cmat(p.i, p.j + 1, p.k), a.q * p.dy1 * p.dz2);
-    add(cmat(p.i, p.j, p.k + 1), a.q * p.dy2 * p.dz1);
-    add(cmat(p.i, p.j + 1, p.k + 1), a.q * p.dy1 * p.dz1);
+    add(cmat(p.i, p.j, p.k), q * p.dy2 * p.dz2);
+    add(cmat(p.i, p.j + 1, p.k), q * p.dy1 * p.dz2);
+    add(cmat(p.i, p.j, p.k + 1), q * p.dy2 * p.dz1);
+    add(cmat(p.i, p.j + 1, p.k + 1), q * p.dy1 * p.dz1);
   }
   __device__ void addy(const DPos& p) const {
-    add(cmat(p.i, p.j, p.k), a.q * p.dx2 * p.dz2);
-    add(cmat(p.i + 1, p.j, p.k), a.q * p.dx1 * p.dz2);
-    add(cmat(p.i, p.j, p.k + 1), a.q * p.dx2 * p.dz1);
-    add(cmat(p.i + 1, p.j, p.k + 1), a.q * p.dx1 * p.dz1);
+    add(cmat(p.i, p.j, p.k), q * p.dx2 * p.dz2);
+    add(cmat(p.i + 1, p.j, p.k), q * p.dx1 * p.dz2);
+    add(cmat(p.i, p.j, p.k + 1), q * p.dx2 * p.dz1);
+    add(cmat(p.i + 1, p.j, p.k + 1), q * p.dx1 * p.dz1);
   }
   __device__ void addz(const DPos& p) const {
-    add(cmat(p.i, p.j, p.k), a.q * p.dx2 * p.dy2);
-    add(cmat(p.i + 1, p.j, p.k), a.q * p.dx1 * p.dy2);
-    add(cmat(p.i, p.j + 1, p.k), a.q * p.dx2 * p.dy1);
-    add(cmat(p.i + 1, p.j + 1, p.k), a.q * p.dx1 * p.dy1);
+    add(cmat(p.i, p.j, p.k), q * p.dx2 * p.dy2);
+    add(cmat(p.i + 1, p.j, p.k), q * p.dx1 * p.dy2);
+    add(cmat(p.i, p.j + 1, p.k), q * p.dx2 * p.dy1);
+    add(cmat(p.i + 1, p.j + 1, p.k), q * p.dx1 * p.dy1);
   }
 };
 
@@ -156,8 +163,7 @@ __device__ __forceinline__ double move_ratio(double d1, double d2, double v) {
 // Spacecraft::distributeInnerParticleCharge with Particle::computeCrossPoints
 // (particle.cpp:82-204): the charge goes to the four cmat nodes of the first
 // crossed face whose orientation matches the velocity sign.
-template <class R>
-__device__ void absorb(const Cond<R>& cd, double x, double y, double z, double vx, double vy, double vz) {
+__device__ void absorb(const Cond& cd, double x, double y, double z, double vx, double vy, double vz) {
   DPos pos, old;
   set_xyz(pos, x, y, z);
   set_xyz(old, x - vx, y - vy, z - vz);
@@ -238,7 +244,19 @@ __device__ void absorb(const Cond<R>& cd, double x, double y, double z, double v
       break;
     }
   }
-  atomicAdd(cd.a.obj[cd.o].absorbed, 1ull);
+  atomicAdd(cd.absorbed, 1ull);
+}
+
+// Surface charge of the particles captured in this step (one thread each).
+__global__ void k_capture(const double* rec, int n, int cx, int cy, int cz, double q, const uint8_t* mask,
+                          ObjArgs o0, ObjArgs o1, ObjArgs o2, ObjArgs o3) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* r = rec + (size_t)8 * i;
+  const int o = (int)r[6];
+  const ObjArgs oa = o == 0 ? o0 : (o == 1 ? o1 : (o == 2 ? o2 : o3));
+  Cond cd{cx, cy, cz, o, q, mask, oa.node_cmat, oa.tmp, oa.absorbed};
+  absorb(cd, r[0], r[1], r[2], r[3], r[4], r[5]);
 }
 
 // Ranks of two predicates over the block, in thread order.
@@ -347,20 +365,29 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   const int ocap = ((MODE & M_BIN) != 0) ? a.ocapa[t] : 0;
   const int rcap = a.capa[t];
 
+  // the next chunk is loaded while the current one is processed
+  auto fetch = [&](int idx, R* q) {
+    if (idx < nreg) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) q[c] = a.reg.c[c][base + idx];
+    } else if (idx < ntot) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) q[c] = (R)a.in[c][ib0 + idx - nreg];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) q[c] = (R)0;
+    }
+  };
+  R nxt[6];
+  fetch(threadIdx.x, nxt);
   for (int c0 = 0; c0 < ntot; c0 += kBlock) {
     const int idx = c0 + threadIdx.x;
     const bool act = idx < ntot;
     const bool from_in = idx >= nreg;
-    R p[6] = {0, 0, 0, 0, 0, 0};
-    if (act) {
-      if (!from_in) {
+    R p[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) p[c] = a.reg.c[c][base + idx];
-      } else {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) p[c] = (R)a.in[c][ib0 + idx - nreg];
-      }
-    }
+    for (int c = 0; c < 6; ++c) p[c] = nxt[c];
+    fetch(idx + kBlock, nxt);
     bool valid = act && !isnan(p[0]);
 
     // ---- velocity: trilinear gather of nodal E, Boris rotation with t = b
@@ -457,8 +484,16 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         const int mi = ci + 1, mj = cj + 1, mk = ck + 1;
         const uint8_t m = a.mask[((size_t)mi * (a.cy - 1) + mj) * (a.cz - 1) + mk];
         if (m != 0) {
-          Cond<R> cd{a, (int)m - 1};
-          absorb(cd, (double)p[0], (double)p[1], (double)p[2], (double)p[3], (double)p[4], (double)p[5]);
+          // captured: recorded for the capture kernel (surface charge on the device)
+          const int slot = atomicAdd(&a.status[3], 1);
+          if (slot < a.cap_cap) {
+            double* r = a.cap_rec + (size_t)8 * slot;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) r[c] = (double)p[c];
+            r[6] = (double)(m - 1);
+          } else {
+            atomicAdd(&a.status[1], 1);
+          }
           valid = false;
         }
       }
@@ -883,6 +918,8 @@ TileArgs<R> tile_args(Ctx& c, int s) {
     a.obj[o].tmp = c.obj[o].tmp_charge.p ? c.obj[o].tmp_charge.p + (size_t)s * c.obj[o].ncmat : nullptr;
     a.obj[o].absorbed = c.obj[o].absorbed.p ? c.obj[o].absorbed.p + s : nullptr;
   }
+  a.cap_rec = c.cap_rec.p + (size_t)8 * c.cap_cap * s;
+  a.cap_cap = c.cap_cap;
   return a;
 }
 
@@ -1053,14 +1090,29 @@ void launch_tiles(Ctx& c, int s) {
 
 template <class R>
 void after_bin(Ctx& c) {
-  // strays -> full re-bin of that species
   std::vector<int32_t> stat(4 * c.S);
   FOPIS_HIP(hipMemcpyAsync(stat.data(), c.status.p, 4 * c.S * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
   FOPIS_HIP(hipStreamSynchronize(c.stream));
+  for (int s = 0; s < c.S; ++s)
+    if (stat[4 * s + 1] > 0) throw Error(FOPIS_E_NOMEM, "stray or capture buffer overflow: particles were lost");
+  // surface charge of the captured particles (Spacecraft::distributeInnerParticleCharge)
   for (int s = 0; s < c.S; ++s) {
-    if (stat[4 * s + 1] > 0) throw Error(FOPIS_E_NOMEM, "stray buffer overflow: particles were lost");
-    if (stat[4 * s + 0] > 0) rebin<R>(c, s);
+    const int n = stat[4 * s + 3];
+    if (n <= 0 || c.nobj == 0) continue;
+    ObjArgs oa[FOPIS_MAX_OBJECTS] = {};
+    for (int o = 0; o < c.nobj; ++o) {
+      oa[o].node_cmat = c.obj[o].node_cmat.p;
+      oa[o].tmp = c.obj[o].tmp_charge.p + (size_t)s * c.obj[o].ncmat;
+      oa[o].absorbed = c.obj[o].absorbed.p + s;
+    }
+    const fopis_species_desc& sd = c.d.species[s];
+    k_capture<<<div_up(n, 128), 128, 0, c.stream>>>(c.cap_rec.p + (size_t)8 * c.cap_cap * s, n, c.cx, c.cy, c.cz,
+                                                    sd.charge * sd.size, c.mask.p, oa[0], oa[1], oa[2], oa[3]);
+    FOPIS_HIP(hipGetLastError());
   }
+  // strays -> full re-bin of that species
+  for (int s = 0; s < c.S; ++s)
+    if (stat[4 * s + 0] > 0) rebin<R>(c, s);
 }
 
 template <class R>
@@ -1108,6 +1160,8 @@ void particles_init(Ctx& c) {
   // rho_part holds S*ntiles windows of R (FP32 windows use half the bytes)
   c.rho_part.alloc((size_t)c.S * c.ntiles * (kRW * kRW * kRW));
   c.dens_part.alloc((size_t)c.S * c.ntiles * (kDW * kDW * kDW));
+  c.cap_cap = 1 << 20;  // captured particles per species and step
+  c.cap_rec.alloc((size_t)8 * c.cap_cap * c.S);
   c.status.alloc(4 * FOPIS_MAX_SPECIES);
   FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), c.stream));
   for (int s = 0; s < c.S; ++s) {
