@@ -83,7 +83,19 @@ def main():
     ap.add_argument("--ppc", type=int, default=0, help="override particles per cell (0 = config)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cmatrix-cache", default="")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -157,7 +169,9 @@ def main():
     }
     tr = pmc_traffic(args.workload)
     if tr:
-        line["roofline"]["traffic"] = tr.get("bytes_per_step")
+        # HBM bytes of one k_tile launch (one species) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+        line["roofline"]["traffic"] = tr.get("bytes_per_launch")
+        line["roofline"]["algorithmic_bytes_per_launch"] = bpp * n_avg / h.S
     h.close()
     if not args.no_cpu_baseline:
         try:
