@@ -135,6 +135,7 @@ def main():
     red_ms, _ = ctx.phase_time("reduce")
     poi_ms, n_poi = ctx.phase_time("poisson")
     ef_ms, _ = ctx.phase_time("efield")
+    stats = F.solver_stats(ctx)
     n_avg = 0.5 * (nparts + n_end)
     value = n_avg / (part_ms * 1e-3)
     bpp = wl["bytes_per_particle"][h.desc.precision]
@@ -160,6 +161,7 @@ def main():
         "reduce_ms": red_ms,
         "poisson_ms": poi_ms,
         "poisson_solves_per_step": n_poi / max(1, n_part),
+        "poisson_cycles_per_solve": stats["total_cycles"] / max(1, stats["total_solves"]),
         "efield_ms": ef_ms,
         "host_generate_ms_per_step": gen_ms / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -409,6 +409,20 @@ int fopis_reset_timers(fopis_ctx* ctx) {
   return guarded([&] {
     timers_collect(ctx->c);
     ctx->c.timers.clear();
+    ctx->c.mg_cycles_total = 0;
+    ctx->c.mg_solves_total = 0;
+  });
+}
+
+int fopis_solver_stats(fopis_ctx* ctx, int32_t* last_cycles, double* last_relres, int64_t* total_cycles,
+                       int64_t* total_solves) {
+  NEED_CTX
+  return guarded([&] {
+    const Ctx& c = ctx->c;
+    if (last_cycles) *last_cycles = c.mg_cycles_last;
+    if (last_relres) *last_relres = c.relres_last;
+    if (total_cycles) *total_cycles = c.mg_cycles_total;
+    if (total_solves) *total_solves = c.mg_solves_total;
   });
 }
 

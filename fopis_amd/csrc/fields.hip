@@ -206,12 +206,9 @@ void fields_reduce_windows(Ctx& c, bool do_rho, bool do_dens) {
   hipStream_t st = c.stream;
   if (do_rho) {
     const int64_t nn = (int64_t)c.nnode;
-    if (c.fp32)
-      k_reduce_rho<float><<<div_up(nn, 256), 256, 0, st>>>(reinterpret_cast<const float*>(c.rho_part.p), c.rho.p, c.S,
-                                                          c.ntiles, c.cx, c.cy, c.cz, c.nt[0], c.nt[1], c.nt[2]);
-    else
-      k_reduce_rho<double><<<div_up(nn, 256), 256, 0, st>>>(c.rho_part.p, c.rho.p, c.S, c.ntiles, c.cx, c.cy, c.cz,
-                                                           c.nt[0], c.nt[1], c.nt[2]);
+    // charge windows are FP64 for both particle precisions
+    k_reduce_rho<double><<<div_up(nn, 256), 256, 0, st>>>(c.rho_part.p, c.rho.p, c.S, c.ntiles, c.cx, c.cy, c.cz,
+                                                         c.nt[0], c.nt[1], c.nt[2]);
   }
   if (do_dens) {
     k_reduce_dens<<<div_up((int64_t)c.ncell_d, 256), 256, 0, st>>>(c.dens_part.p, c.density.p, sizes_buf(c).p, c.S,

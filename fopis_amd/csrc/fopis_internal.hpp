@@ -189,6 +189,7 @@ struct Ctx {
 
   std::vector<MgLevel> mg;
   int mg_cycles_last = 0;
+  int64_t mg_cycles_total = 0, mg_solves_total = 0;
   double relres_last = 0;
 
   std::map<std::string, PhaseTimer> timers;

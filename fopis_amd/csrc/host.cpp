@@ -783,11 +783,15 @@ StepBatch GpuRootGrid::newBatch() const {
   return b;
 }
 
+// Each species draws from its own engines, so species run on separate threads
+// and the draws stay in the reference's per-stream order.
 void GpuRootGrid::injectInto(StepBatch& b) {
-  for (size_t s = 0; s < sp_.size(); ++s) {
-    auto ps = generateInjection((int)s);
-    b.particles[s].insert(b.particles[s].end(), ps.begin(), ps.end());
-  }
+  std::vector<std::vector<Particle6>> out(sp_.size());
+  std::vector<std::thread> th;
+  for (size_t s = 0; s < sp_.size(); ++s)
+    if (sp_[s].type == "ambient") th.emplace_back([this, s, &out] { out[s] = generateInjection((int)s); });
+  for (auto& t : th) t.join();
+  for (size_t s = 0; s < sp_.size(); ++s) b.particles[s].insert(b.particles[s].end(), out[s].begin(), out[s].end());
 }
 
 void GpuRootGrid::stageBatch(StepBatch& b) {

@@ -51,7 +51,7 @@ struct TileArgs {
   SoaPtr<R> stray;
   int64_t stray_cap;
   int32_t* status;  // [0] strays, [1] lost, [2] out-of-window deposits
-  R* rho_part;
+  double* rho_part;
   int32_t* dens_part;
   double* rho_g;
   double* dens_g;
@@ -303,10 +303,17 @@ struct TileSmem {
   static constexpr bool kE = (MODE & M_VEL) != 0;
   static constexpr bool kR = (MODE & M_DEP) != 0;
   static constexpr bool kD = (MODE & (M_DEP | M_DENS)) != 0;
-  static constexpr size_t e_bytes = kE ? sizeof(R) * 3 * kEW * kEW * kEW : 0;
-  static constexpr size_t r_bytes = kR ? sizeof(R) * kRW * kRW * kRW : 0;
-  static constexpr size_t d_bytes = kD ? sizeof(int) * kDW * kDW * kDW : 0;
+  // every segment starts on a 16-byte boundary: the FP64 charge window must be
+  // 8-byte aligned in LDS whatever the size of the E window before it
+  static constexpr size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+  static constexpr size_t e_bytes = kE ? align16(sizeof(R) * 3 * kEW * kEW * kEW) : 0;
+  // charge windows accumulate in FP64 for both particle precisions: FP32 LDS
+  // atomics saturate the LDS data FIFO on gfx950 (SQ_WAIT_INST_LDS 50x the
+  // FP64 kernel's), FP64 ones do not
+  static constexpr size_t r_bytes = kR ? align16(sizeof(double) * kRW * kRW * kRW) : 0;
+  static constexpr size_t d_bytes = kD ? align16(sizeof(int) * kDW * kDW * kDW) : 0;
   static constexpr size_t bytes = e_bytes + r_bytes + d_bytes + 16;
+  static_assert(e_bytes % 16 == 0 && r_bytes % 16 == 0, "LDS segments must stay 16-byte aligned");
 };
 
 // One workgroup per tile: stage the tile's E nodes in LDS, stream its
@@ -319,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   __shared__ __attribute__((aligned(16))) char smem[SM::bytes];
   __shared__ int sh_rank[8];
   R* sE = reinterpret_cast<R*>(smem);
-  R* sR = reinterpret_cast<R*>(smem + SM::e_bytes);
+  double* sR = reinterpret_cast<double*>(smem + SM::e_bytes);
   int* sD = reinterpret_cast<int*>(smem + SM::e_bytes + SM::r_bytes);
 
   const int t = blockIdx.x;
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     }
   }
   if constexpr (SM::kR) {
-    for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) sR[q] = (R)0;
+    for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) sR[q] = 0.0;
   }
   if constexpr (SM::kD) {
     for (int q = threadIdx.x; q < kDW * kDW * kDW; q += kBlock) sD[q] = 0;
@@ -506,15 +513,15 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
                         dx2 * dy2 * dz1 * q, dx1 * dy2 * dz1 * q, dx2 * dy1 * dz1 * q, dx1 * dy1 * dz1 * q};
         const int wi = di - o0 + 1, wj = dj - o1 + 1, wk = dk - o2 + 1;
         if (wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW) {
-          R* W = sR + (wi * kRW + wj) * kRW + wk;
-          atomicAdd(W, w[0]);
-          atomicAdd(W + kRW * kRW, w[1]);
-          atomicAdd(W + kRW, w[2]);
-          atomicAdd(W + kRW * kRW + kRW, w[3]);
-          atomicAdd(W + 1, w[4]);
-          atomicAdd(W + kRW * kRW + 1, w[5]);
-          atomicAdd(W + kRW + 1, w[6]);
-          atomicAdd(W + kRW * kRW + kRW + 1, w[7]);
+          double* W = sR + (wi * kRW + wj) * kRW + wk;
+          atomicAdd(W, (double)w[0]);
+          atomicAdd(W + kRW * kRW, (double)w[1]);
+          atomicAdd(W + kRW, (double)w[2]);
+          atomicAdd(W + kRW * kRW + kRW, (double)w[3]);
+          atomicAdd(W + 1, (double)w[4]);
+          atomicAdd(W + kRW * kRW + 1, (double)w[5]);
+          atomicAdd(W + kRW + 1, (double)w[6]);
+          atomicAdd(W + kRW * kRW + kRW + 1, (double)w[7]);
         } else {
           const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
           double* G = a.rho_g + (size_t)(di + 1) * sI + (size_t)(dj + 1) * sJ + (dk + 1);
@@ -587,7 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   }
   if constexpr (SM::kR || SM::kD) __syncthreads();
   if constexpr (SM::kR) {
-    R* P = a.rho_part + (size_t)t * (kRW * kRW * kRW);
+    double* P = a.rho_part + (size_t)t * (kRW * kRW * kRW);
     for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) P[q] = sR[q];
   }
   if constexpr (SM::kD) {
@@ -907,7 +914,7 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.stray = ptrs(b.stray);
   a.stray_cap = b.stray_cap;
   a.status = c.status.p + 4 * s;
-  a.rho_part = reinterpret_cast<R*>(c.rho_part.p) + (size_t)s * c.ntiles * (kRW * kRW * kRW);
+  a.rho_part = c.rho_part.p + (size_t)s * c.ntiles * (kRW * kRW * kRW);
   a.dens_part = c.dens_part.p + (size_t)s * c.ntiles * (kDW * kDW * kDW);
   a.rho_g = c.rho.p + (size_t)(s + 1) * c.nnode;
   a.dens_g = c.density.p + (size_t)s * c.ncell_d;
@@ -1157,7 +1164,7 @@ void particles_init(Ctx& c) {
     c.sp32.resize(c.S);
   else
     c.sp64.resize(c.S);
-  // rho_part holds S*ntiles windows of R (FP32 windows use half the bytes)
+  // rho_part holds S*ntiles FP64 charge windows (both particle precisions)
   c.rho_part.alloc((size_t)c.S * c.ntiles * (kRW * kRW * kRW));
   c.dens_part.alloc((size_t)c.S * c.ntiles * (kDW * kDW * kDW));
   c.cap_cap = 1 << 20;  // captured particles per species and step

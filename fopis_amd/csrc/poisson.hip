@@ -56,15 +56,27 @@ __global__ __launch_bounds__(256) void k_resid0(const double* __restrict__ phi, 
   }
 }
 
-__global__ void k_sum2(const double* part, int nb, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double a = 0.0, b = 0.0;
-    for (int i = 0; i < nb; ++i) {
-      a += part[2 * i];
-      b += part[2 * i + 1];
+// Final sum of the block partials: one block, fixed tree (deterministic).
+__global__ __launch_bounds__(256) void k_sum2(const double* part, int nb, double* out) {
+  __shared__ double s1[256], s2[256];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
     }
-    out[0] = a;
-    out[1] = b;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s1[0];
+    out[1] = s2[0];
   }
 }
 
@@ -261,7 +273,7 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   double nrm[2];
   auto residual = [&]() {
     k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, c.red.p);
-    k_sum2<<<1, 1, 0, c.stream>>>(c.red.p, nb, c.red.p + 2 * nb);
+    k_sum2<<<1, 256, 0, c.stream>>>(c.red.p, nb, c.red.p + 2 * nb);
     FOPIS_HIP(hipMemcpyAsync(nrm, c.red.p + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     FOPIS_HIP(hipStreamSynchronize(c.stream));
   };
@@ -353,6 +365,8 @@ int poisson_solve(Ctx& c, double tol, int max_cycles, double* relres, const doub
   else
     cyc = solve_impl<double>(c, tol, max_cycles, relres, rho0);
   c.mg_cycles_last = cyc;
+  c.mg_cycles_total += cyc;
+  c.mg_solves_total += 1;
   c.relres_last = *relres;
   return cyc;
 }

@@ -26,7 +26,7 @@ EXPORTS = [
     "fopis_main_loop_es", "fopis_initialize", "fopis_get_field", "fopis_set_field", "fopis_energy",
     "fopis_set_object", "fopis_build_cmatrix", "fopis_get_cmatrix", "fopis_set_cmatrix",
     "fopis_add_object_charge", "fopis_get_object_state", "fopis_set_object_charge_map", "fopis_phase_time",
-    "fopis_reset_timers", "fopis_synchronize",
+    "fopis_reset_timers", "fopis_synchronize", "fopis_solver_stats", "fopis_load_maxwellian",
 ]
 
 
@@ -292,3 +292,19 @@ class Context:
 
     def reset_timers(self):
         self._ok(self.L.fopis_reset_timers(self.h))
+
+
+def _bind_stats(L):
+    if getattr(L, "_stats_bound", False):
+        return
+    L.fopis_solver_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L._stats_bound = True
+
+
+def solver_stats(ctx: "Context") -> dict:
+    """Poisson cycles/relres of the last solve and totals since reset_timers."""
+    _bind_stats(ctx.L)
+    lc, lr, tc, ts = C.c_int32(), C.c_double(), C.c_int64(), C.c_int64()
+    ctx._ok(ctx.L.fopis_solver_stats(ctx.h, C.byref(lc), C.byref(lr), C.byref(tc), C.byref(ts)))
+    return {"last_cycles": lc.value, "last_relres": lr.value, "total_cycles": tc.value, "total_solves": ts.value}

@@ -195,6 +195,11 @@ int fopis_set_object_charge_map(fopis_ctx* ctx, int obj, const double* charge_ma
  * "poisson", "efield", "bin". n = number of samples. */
 int fopis_phase_time(fopis_ctx* ctx, const char* phase, double* avg_ms, int64_t* n);
 int fopis_reset_timers(fopis_ctx* ctx);
+/* Poisson statistics: cycles and relres of the last solve, totals since the
+ * last fopis_reset_timers (the reference prints the SOR count per solve,
+ * RootGridField.cpp:86-88). Any pointer may be NULL. */
+int fopis_solver_stats(fopis_ctx* ctx, int32_t* last_cycles, double* last_relres, int64_t* total_cycles,
+                       int64_t* total_solves);
 /* Blocks until the context's stream is idle. */
 int fopis_synchronize(fopis_ctx* ctx);
 

@@ -255,3 +255,26 @@ def test_fp32_push_close_to_fp64():
         # ulps of 2^-19; velocities agree to FP32 relative precision
         assert np.max(np.abs(a[valid, :3] - b[valid, :3])) < 4 * 2.0 ** -19
         assert np.max(np.abs(a[valid, 3:] - b[valid, 3:])) < 1e-6
+
+
+def test_fp32_fused_step_deposit():
+    """The FP32 fused kernel (push + move + count + deposit + bin in one launch)
+    deposits the charge of the particles it moved: the oracle's CIC deposit of
+    the same post-step particles agrees to FP32 weight rounding (the windows
+    accumulate in FP64), and the density counts agree exactly."""
+    g = _oracle(ppc=6, b=(0.0, 0.0, 2028.6))
+    ctx = _ctx(g, precision=F.FP32)
+    _upload(ctx, g)
+    ctx.initialize()
+    ctx.main_loop_es()
+    for s in range(g.S):
+        d = ctx.get_particles(s)
+        keep = ~np.isnan(np.asarray(d["x"]))
+        g.set_particles(s, {k: np.asarray(d[k], dtype=np.float64)[keep] for k in ("x", "y", "z", "vx", "vy", "vz")})
+    g.call("rc_update_density")
+    g.call("rc_deposit_only")
+    for s in range(g.S):
+        np.testing.assert_array_equal(ctx.get_field(F.DENSITY, s), g.get_field(refcpu.DENSITY, s))
+        a, b = ctx.get_field(F.RHO, s + 1), g.get_field(refcpu.RHO, s + 1)
+        # each particle's eight FP32 weights carry ~3 roundings of 2^-24
+        assert np.max(np.abs(a - b)) <= 2e-6 * np.max(np.abs(b)), s
