@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfopis_hip.so")
 ARCH = os.environ.get("FOPIS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["capi.cpp", "particles.hip", "fields.hip", "poisson.hip", "objects.hip", "radix.hip", "host.cpp"]
+SOURCES = ["capi.cpp", "particles.hip", "fields.hip", "poisson.hip", "objects.hip", "radix.hip", "guess.hip", "host.cpp"]
 
 COMMON = [
     "-O3",

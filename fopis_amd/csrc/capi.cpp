@@ -64,11 +64,14 @@ static void check_desc(const fopis_desc& d) {
   if (d.mg_precision != FOPIS_FP64 && d.mg_precision != FOPIS_FP32) throw Error(FOPIS_E_ARG, "bad mg precision");
 }
 
+// With the learned guess on, the first Poisson solve of a step starts from
+// the network's potential instead of the previous step's.
 static void update_rho(Ctx& c, bool push) {
   particles_deposit_bin(c, push);
   objects_sum_whole_charge(c);
   if (c.nobj > 0) {
     double rr = 0.0;
+    if (guess_mode(c) == 1) guess_apply(c);
     poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
     objects_redistribute(c);
   }
@@ -175,6 +178,7 @@ int fopis_destroy(fopis_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->c.stream);
   timers_collect(ctx->c);
   hipStream_t st = ctx->c.stream;
+  guess_free(ctx->c);
   delete ctx;
   if (st) (void)hipStreamDestroy(st);
   return FOPIS_OK;
@@ -298,6 +302,7 @@ int fopis_main_loop_es(fopis_ctx* ctx) {
     Ctx& c = ctx->c;
     update_rho(c, true);
     double rr = 0.0;
+    if (c.nobj == 0 && guess_mode(c) == 1) guess_apply(c);
     poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
     if (!(rr <= c.d.poisson_tol)) throw Error(FOPIS_E_NOCONV, "Poisson solve stopped at relres " + std::to_string(rr));
     fields_efield(c);
@@ -424,6 +429,132 @@ int fopis_solver_stats(fopis_ctx* ctx, int32_t* last_cycles, double* last_relres
     if (total_cycles) *total_cycles = c.mg_cycles_total;
     if (total_solves) *total_solves = c.mg_solves_total;
   });
+}
+
+int fopis_guess_init(fopis_ctx* ctx, uint64_t seed) {
+  NEED_CTX
+  return guarded([&] { guess_init(ctx->c, seed); });
+}
+
+int fopis_guess_num_params(fopis_ctx* ctx, int64_t* n) {
+  NEED_CTX
+  if (!n) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { *n = guess_num_params(ctx->c); });
+}
+
+int fopis_guess_get_params(fopis_ctx* ctx, float* params, double* out_scale) {
+  NEED_CTX
+  if (!params) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { guess_get_params(ctx->c, params, out_scale); });
+}
+
+int fopis_guess_set_params(fopis_ctx* ctx, const float* params, double out_scale) {
+  NEED_CTX
+  if (!params) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { guess_set_params(ctx->c, params, out_scale); });
+}
+
+int fopis_guess_set_mode(fopis_ctx* ctx, int mode) {
+  NEED_CTX
+  return guarded([&] { guess_set_mode(ctx->c, mode); });
+}
+
+int fopis_guess_apply(fopis_ctx* ctx) {
+  NEED_CTX
+  return guarded([&] { guess_apply(ctx->c); });
+}
+
+int fopis_guess_dataset_alloc(fopis_ctx* ctx, int32_t capacity) {
+  NEED_CTX
+  return guarded([&] { guess_dataset_alloc(ctx->c, capacity); });
+}
+
+int fopis_guess_dataset_store(fopis_ctx* ctx, int32_t slot, double* phi_hat_rms) {
+  NEED_CTX
+  return guarded([&] {
+    const double r = guess_dataset_store(ctx->c, slot);
+    if (phi_hat_rms) *phi_hat_rms = r;
+  });
+}
+
+int fopis_guess_dataset_get(fopis_ctx* ctx, int32_t slot, uint16_t* rho_hat, uint8_t* mask, uint16_t* phi_hat) {
+  NEED_CTX
+  return guarded([&] { guess_dataset_get(ctx->c, slot, rho_hat, mask, phi_hat); });
+}
+
+int fopis_guess_dataset_set(fopis_ctx* ctx, int32_t slot, const uint16_t* rho_hat, const uint8_t* mask,
+                            const uint16_t* phi_hat) {
+  NEED_CTX
+  if (!rho_hat || !mask || !phi_hat) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { guess_dataset_set(ctx->c, slot, rho_hat, mask, phi_hat); });
+}
+
+int fopis_guess_grad(fopis_ctx* ctx, int32_t slot, double* loss, float* grad) {
+  NEED_CTX
+  return guarded([&] {
+    const double l = guess_grad(ctx->c, slot, grad);
+    if (loss) *loss = l;
+  });
+}
+
+int fopis_guess_train(fopis_ctx* ctx, int32_t npairs, int32_t epochs, double lr, uint64_t seed, double* epoch_loss) {
+  NEED_CTX
+  return guarded([&] { guess_train(ctx->c, npairs, epochs, lr, seed, epoch_loss); });
+}
+
+int fopis_guess_eval(fopis_ctx* ctx, int32_t begin, int32_t end, double* loss) {
+  NEED_CTX
+  if (!loss) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { *loss = guess_eval(ctx->c, begin, end); });
+}
+
+int fopis_guess_synth_boxes(fopis_ctx* ctx, int32_t nbox, const int32_t* lohi, const double* charge, uint64_t seed) {
+  NEED_CTX
+  if (nbox > 0 && (!lohi || !charge)) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] {
+    if (nbox == 0)
+      guess_clear_boxes(ctx->c);
+    else
+      guess_synth_boxes(ctx->c, nbox, lohi, charge, seed);
+  });
+}
+
+int fopis_nn_conv3d(int32_t n0, int32_t n1, int32_t n2, int32_t cin, int32_t cout, int32_t up, int32_t relu,
+                    const uint16_t* x, const float* w, const float* b, const uint16_t* add, uint16_t* y, int32_t reps,
+                    float* ms) {
+  if (!x || !w || !b || !y) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { nn_conv3d(n0, n1, n2, cin, cout, up, relu, x, w, b, add, y, reps, ms); });
+}
+
+int fopis_nn_conv3d_backward(int32_t n0, int32_t n1, int32_t n2, int32_t cin, int32_t cout, int32_t up,
+                             const uint16_t* x, const float* w, const uint16_t* gy, float* dw, float* db,
+                             uint16_t* dx) {
+  if (!x || !w || !gy || !dw || !db) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { nn_conv3d_backward(n0, n1, n2, cin, cout, up, x, w, gy, dw, db, dx); });
 }
 
 int fopis_synchronize(fopis_ctx* ctx) {

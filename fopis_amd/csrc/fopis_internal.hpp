@@ -155,6 +155,8 @@ struct MgLevel {
   DBuf<float> e32, r32, t32;   // FP32 level arrays
 };
 
+struct GuessNet;  // learned initial guess (guess.hip)
+
 struct Ctx {
   fopis_desc d{};
   int S = 0;
@@ -191,6 +193,13 @@ struct Ctx {
   int mg_cycles_last = 0;
   int64_t mg_cycles_total = 0, mg_solves_total = 0;
   double relres_last = 0;
+  DBuf<int32_t> mg_state;     // [4] solve flag and cycle count on the device
+  int mg_hist[2] = {1, 1};    // cycles of the last two solves (batch size of the next)
+  int mg_tail = 0;            // first level solved by the single-workgroup LDS kernel
+  size_t mg_tail_bytes = 0;   // its LDS bytes
+  size_t mg_tail_esz = 0;     // element size the tail was planned for
+  size_t lds_max = 65536;     // LDS bytes per workgroup on the device
+  GuessNet* guess = nullptr;  // owned; released by guess_free
 
   std::map<std::string, PhaseTimer> timers;
   bool timing = true;
@@ -234,6 +243,28 @@ void objects_redistribute(Ctx& c);
 void objects_add_charge(Ctx& c, int o, const double* dq, const double* dcur);
 void objects_state(Ctx& c, int o, double* pot, double* q, double* cur, double* cmap);
 void objects_set_charge_map(Ctx& c, int o, const double* cmap);
+
+void guess_init(Ctx& c, uint64_t seed);
+void guess_free(Ctx& c);
+int64_t guess_num_params(Ctx& c);
+void guess_get_params(Ctx& c, float* out, double* k_out);
+void guess_set_params(Ctx& c, const float* in, double k_out);
+void guess_set_mode(Ctx& c, int mode);
+int guess_mode(const Ctx& c);
+void guess_apply(Ctx& c);
+void guess_dataset_alloc(Ctx& c, int capacity);
+double guess_dataset_store(Ctx& c, int slot);
+void guess_dataset_get(Ctx& c, int slot, uint16_t* rho_hat, uint8_t* mask, uint16_t* phi_hat);
+void guess_dataset_set(Ctx& c, int slot, const uint16_t* rho_hat, const uint8_t* mask, const uint16_t* phi_hat);
+double guess_grad(Ctx& c, int slot, float* grad_out);
+void guess_train(Ctx& c, int nsamples, int epochs, double lr, uint64_t seed, double* loss_out);
+double guess_eval(Ctx& c, int begin, int end);
+void guess_synth_boxes(Ctx& c, int nbox, const int32_t* lohi, const double* q, uint64_t seed);
+void guess_clear_boxes(Ctx& c);
+void nn_conv3d(int n0, int n1, int n2, int cin, int cout, int up, int relu, const uint16_t* x, const float* w,
+               const float* b, const uint16_t* add, uint16_t* y, int reps, float* ms);
+void nn_conv3d_backward(int n0, int n1, int n2, int cin, int cout, int up, const uint16_t* x, const float* w,
+                        const uint16_t* gy, float* dw_out, float* db_out, uint16_t* dx_out);
 
 void timer_begin(Ctx& c, const char* name, hipEvent_t* ev);
 void timer_end(Ctx& c, const char* name, hipEvent_t ev);

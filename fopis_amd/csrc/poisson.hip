@@ -8,6 +8,16 @@
 // current values (0 on Dirichlet faces), then applies the face conditions
 // (Field.cpp:10-96). The reference reaches it by zebra SOR; this build stops
 // on ||b - A phi||_2 / ||b||_2 <= tol in FP64.
+//
+// Launch structure. The host queues cycles in batches without waiting: every
+// kernel of a cycle first reads a device flag that the norm kernel sets once
+// the tolerance (or the cycle cap) is reached, and returns at once when it is
+// set, so a batch that overshoots costs only empty launches. The host reads
+// the flag once per batch. The coarse levels whose arrays fit in LDS run as
+// one single-workgroup kernel per cycle.
+#include <algorithm>
+#include <cmath>
+
 #include "fopis_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -21,15 +31,40 @@ struct Lv {
   int64_t sx, sy;
 };
 
+// Workgroups are handed to the 8 XCDs round-robin by linear id, and each XCD
+// has its own L2. Renumber them so that XCD x runs a contiguous run of the
+// logical ids (whole planes of the level), keeping a stencil's neighbour rows
+// in the same L2. The map is a bijection for any grid size.
+__device__ inline int xcd_block(int p, int n) {
+  const int per = n >> 3, rem = n & 7, x = p & 7, slot = p >> 3;
+  return x * per + min(x, rem) + slot;
+}
+
+// (chunk, row, plane) of the calling workgroup in a 3-D launch, XCD-renumbered.
+__device__ inline void block3(int& bx, int& by, int& bz) {
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int n = gx * gy * gridDim.z;
+  const int L = xcd_block(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), n);
+  bx = L % gx;
+  by = (L / gx) % gy;
+  bz = L / (gx * gy);
+}
+
 // b = rc * rho at the level-0 interior; r = b - A phi written as P;
 // per-block partial sums of r^2 and b^2 in FP64.
 template <class P>
 __global__ __launch_bounds__(256) void k_resid0(const double* __restrict__ phi, const double* __restrict__ rho,
-                                                double rc, P* __restrict__ r, Lv L, double* part) {
+                                                double rc, P* __restrict__ r, Lv L, double* part,
+                                                const int* __restrict__ done) {
+  if (*done) return;
   __shared__ double s1[256], s2[256];
   const int64_t nint = (int64_t)L.n0 * L.n1 * L.n2;
+  // each workgroup sums one contiguous range of points (XCD-renumbered)
+  const int64_t chunk = ((nint + (int64_t)gridDim.x * 256 - 1) / ((int64_t)gridDim.x * 256)) * 256;
+  const int64_t q0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk;
+  const int64_t q1 = q0 + chunk < nint ? q0 + chunk : nint;
   double rr = 0.0, bb = 0.0;
-  for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < nint; q += (int64_t)gridDim.x * 256) {
+  for (int64_t q = q0 + threadIdx.x; q < q1; q += 256) {
     const int c = (int)(q % L.n2) + 1, b = (int)((q / L.n2) % L.n1) + 1, a = (int)(q / ((int64_t)L.n1 * L.n2)) + 1;
     const int64_t g = a * L.sx + b * L.sy + c;
     const double bv = rc * rho[g];
@@ -56,8 +91,18 @@ __global__ __launch_bounds__(256) void k_resid0(const double* __restrict__ phi, 
   }
 }
 
-// Final sum of the block partials: one block, fixed tree (deterministic).
-__global__ __launch_bounds__(256) void k_sum2(const double* part, int nb, double* out) {
+// Solve state on the device: st[0] done flag, st[1] cycles applied;
+// dst[0..2] = ||r||^2, ||b||^2, relative residual of the last check.
+__global__ void k_solve_begin(int* st) {
+  st[0] = 0;
+  st[1] = 0;
+}
+
+// Final sum of the residual partials (one block, fixed tree: deterministic),
+// cycle count and the stopping test of the solve loop.
+__global__ __launch_bounds__(256) void k_norm(const double* part, int nb, double* dst, int* st, double tol,
+                                              int max_cycles, int count_cycle) {
+  if (st[0]) return;
   __shared__ double s1[256], s2[256];
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < nb; i += 256) {
@@ -75,38 +120,76 @@ __global__ __launch_bounds__(256) void k_sum2(const double* part, int nb, double
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = s1[0];
-    out[1] = s2[0];
+    const double rr = s1[0], bb = s2[0];
+    const int cyc = st[1] + count_cycle;
+    const double rel = bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+    dst[0] = rr;
+    dst[1] = bb;
+    dst[2] = rel;
+    st[1] = cyc;
+    st[0] = (rel <= tol || (bb == 0.0 && rr < 1e-300) || cyc >= max_cycles) ? 1 : 0;
   }
 }
 
-// Red-black SOR sweep over points with (a+b+c) % 2 == color.
+// Red-black SOR half-sweep over points with (a+b+c) % 2 == color:
+// e = (1-w) e + (w/6) (sum of neighbours + r).
 template <class P>
-__global__ void k_smooth(P* __restrict__ e, const P* __restrict__ r, Lv L, int color, P w) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y + 1, a = blockIdx.z + 1;
+__global__ void k_smooth(P* __restrict__ e, const P* __restrict__ r, Lv L, int color, P w1, P w6,
+                         const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int m = bx * blockDim.x + threadIdx.x;
+  const int b = by + 1, a = bz + 1;
   const int c0 = ((a + b + 1) & 1) == color ? 1 : 2;
   const int c = c0 + 2 * m;
   if (c > L.n2) return;
   const int64_t g = a * L.sx + b * L.sy + c;
   const P nb = e[g - L.sx] + e[g + L.sx] + e[g - L.sy] + e[g + L.sy] + e[g - 1] + e[g + 1];
-  e[g] = ((P)1 - w) * e[g] + w * (nb + r[g]) / (P)6;
+  e[g] = w1 * e[g] + w6 * (nb + r[g]);
+}
+
+// First (red) half-sweep of a cycle from e = 0: red points get (w/6) r and
+// black points are cleared, so the level array needs no separate zeroing.
+template <class P>
+__global__ void k_smooth_first(P* __restrict__ e, const P* __restrict__ r, Lv L, P w6, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int c = bx * blockDim.x + threadIdx.x + 1;
+  const int b = by + 1, a = bz + 1;
+  if (c > L.n2) return;
+  const int64_t g = a * L.sx + b * L.sy + c;
+  e[g] = ((a + b + c) & 1) == 0 ? w6 * r[g] : (P)0;
 }
 
 template <class P>
-__global__ void k_resid(P* __restrict__ t, const P* __restrict__ e, const P* __restrict__ r, Lv L) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int b = blockIdx.y + 1, a = blockIdx.z + 1;
+__global__ void k_resid(P* __restrict__ t, const P* __restrict__ e, const P* __restrict__ r, Lv L,
+                        const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int c = bx * blockDim.x + threadIdx.x + 1;
+  const int b = by + 1, a = bz + 1;
   if (c > L.n2) return;
   const int64_t g = a * L.sx + b * L.sy + c;
   t[g] = r[g] - ((P)6 * e[g] - (e[g - L.sx] + e[g + L.sx] + e[g - L.sy] + e[g + L.sy] + e[g - 1] + e[g + 1]));
 }
 
+template <class P>
+__device__ inline P fw_weight(int da, int db, int dc) {
+  const int nz = (da != 0) + (db != 0) + (dc != 0);
+  return nz == 0 ? (P)1 : (nz == 1 ? (P)0.5 : (nz == 2 ? (P)0.25 : (P)0.125));
+}
+
 // Full weighting to the coarse grid, scaled by 4 for the unscaled 2h operator.
 template <class P>
-__global__ void k_restrict(P* __restrict__ rc, Lv C, const P* __restrict__ t, Lv F) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int b = blockIdx.y + 1, a = blockIdx.z + 1;
+__global__ void k_restrict(P* __restrict__ rc, Lv C, const P* __restrict__ t, Lv F, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int c = bx * blockDim.x + threadIdx.x + 1;
+  const int b = by + 1, a = bz + 1;
   if (c > C.n2) return;
   const int64_t f = (2 * a) * F.sx + (2 * b) * F.sy + 2 * c;
   P acc = 0;
@@ -115,19 +198,18 @@ __global__ void k_restrict(P* __restrict__ rc, Lv C, const P* __restrict__ t, Lv
 #pragma unroll
     for (int db = -1; db <= 1; ++db)
 #pragma unroll
-      for (int dc = -1; dc <= 1; ++dc) {
-        const int nz = (da != 0) + (db != 0) + (dc != 0);
-        const P wgt = nz == 0 ? (P)1 : (nz == 1 ? (P)0.5 : (nz == 2 ? (P)0.25 : (P)0.125));
-        acc += wgt * t[f + da * F.sx + db * F.sy + dc];
-      }
+      for (int dc = -1; dc <= 1; ++dc) acc += fw_weight<P>(da, db, dc) * t[f + da * F.sx + db * F.sy + dc];
   rc[a * C.sx + b * C.sy + c] = acc * (P)0.5;  // (4/8) * sum
 }
 
 // e_fine += trilinear interpolation of e_coarse (coarse node A sits at fine 2A).
 template <class P>
-__global__ void k_prolong(P* __restrict__ e, Lv F, const P* __restrict__ ec, Lv C) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int b = blockIdx.y + 1, a = blockIdx.z + 1;
+__global__ void k_prolong(P* __restrict__ e, Lv F, const P* __restrict__ ec, Lv C, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int c = bx * blockDim.x + threadIdx.x + 1;
+  const int b = by + 1, a = bz + 1;
   if (c > F.n2) return;
   const int a0 = a >> 1, a1 = (a + 1) >> 1, b0 = b >> 1, b1 = (b + 1) >> 1, c0 = c >> 1, c1 = (c + 1) >> 1;
   auto E = [&](int x, int y, int z) { return ec[x * C.sx + y * C.sy + z]; };
@@ -138,25 +220,168 @@ __global__ void k_prolong(P* __restrict__ e, Lv F, const P* __restrict__ ec, Lv 
 }
 
 template <class P>
-__global__ void k_zero(P* __restrict__ e, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) e[i] = (P)0;
+__global__ void k_zero_level(P* __restrict__ e, Lv L, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int c = bx * blockDim.x + threadIdx.x + 1;
+  const int b = by + 1, a = bz + 1;
+  if (c > L.n2) return;
+  e[a * L.sx + b * L.sy + c] = (P)0;
+}
+
+// One unknown: A = 6.
+template <class P>
+__global__ void k_coarse_single(P* e, const P* r, Lv L, const int* __restrict__ done) {
+  if (*done) return;
+  const int64_t g = L.sx + L.sy + 1;
+  e[g] = r[g] / (P)6;
 }
 
 template <class P>
-__global__ void k_add(double* __restrict__ phi, const P* __restrict__ e, Lv L) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int b = blockIdx.y + 1, a = blockIdx.z + 1;
+__global__ void k_add(double* __restrict__ phi, const P* __restrict__ e, Lv L, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int c = bx * blockDim.x + threadIdx.x + 1;
+  const int b = by + 1, a = bz + 1;
   if (c > L.n2) return;
   const int64_t g = a * L.sx + b * L.sy + c;
   phi[g] += (double)e[g];
 }
 
+// ---- coarse tail: the V-cycle below level Lc in one workgroup, arrays in LDS
+
+constexpr int kTailMaxLevels = 8;
+constexpr int kTailThreads = 1024;
+
+struct TailLv {
+  int n0, n1, n2;
+  int sx, sy;    // strides of the padded (n+2)^3 array
+  int size;      // elements
+  int oe, orr, ot;  // element offsets of e, r, t in LDS
+};
+
 template <class P>
-__global__ void k_coarse_single(P* e, const P* r, Lv L) {
-  // one unknown: A = 6
-  const int64_t g = L.sx + L.sy + 1;
-  e[g] = r[g] / (P)6;
+struct TailArgs {
+  int nl;
+  TailLv lv[kTailMaxLevels];
+  P* ge;                // level-Lc e in global memory
+  const P* gr;          // level-Lc r in global memory
+  int64_t gsx, gsy;     // its strides
+  P w1, w6;
+  int pre, post, coarse_sweeps;
+  int total;            // LDS elements
+  const int* done;
+};
+
+// Tail loops give each thread whole (b, c) columns of a level and walk a.
+template <class P>
+__device__ void tail_smooth(P* s, const TailLv& L, int sweeps, P w1, P w6) {
+  const int ncol = L.n1 * L.n2;
+  P* e = s + L.oe;
+  const P* r = s + L.orr;
+  for (int it = 0; it < sweeps; ++it)
+    for (int col = 0; col < 2; ++col) {
+      for (int t = threadIdx.x; t < ncol; t += kTailThreads) {
+        const int b = t / L.n2 + 1, c = t % L.n2 + 1;
+        const int a0 = (col ^ ((b + c) & 1)) ? 1 : 2;  // (a + b + c) % 2 == col
+        for (int a = a0; a <= L.n0; a += 2) {
+          const int g = a * L.sx + b * L.sy + c;
+          const P nb = e[g - L.sx] + e[g + L.sx] + e[g - L.sy] + e[g + L.sy] + e[g - 1] + e[g + 1];
+          e[g] = w1 * e[g] + w6 * (nb + r[g]);
+        }
+      }
+      __syncthreads();
+    }
+}
+
+template <class P>
+__global__ __launch_bounds__(kTailThreads) void k_coarse_tail(TailArgs<P> A) {
+  if (*A.done) return;
+  extern __shared__ __attribute__((aligned(16))) char tail_lds[];
+  P* s = reinterpret_cast<P*>(tail_lds);
+  for (int q = threadIdx.x; q < A.total; q += kTailThreads) s[q] = (P)0;
+  __syncthreads();
+  {
+    const TailLv& L = A.lv[0];
+    for (int t = threadIdx.x; t < L.n1 * L.n2; t += kTailThreads) {
+      const int b = t / L.n2 + 1, c = t % L.n2 + 1;
+      for (int a = 1; a <= L.n0; ++a) s[L.orr + a * L.sx + b * L.sy + c] = A.gr[a * A.gsx + b * A.gsy + c];
+    }
+    __syncthreads();
+  }
+  // descend
+  for (int l = 0; l + 1 < A.nl; ++l) {
+    const TailLv& F = A.lv[l];
+    const TailLv& C = A.lv[l + 1];
+    tail_smooth<P>(s, F, A.pre, A.w1, A.w6);
+    const P* e = s + F.oe;
+    for (int t = threadIdx.x; t < F.n1 * F.n2; t += kTailThreads) {
+      const int b = t / F.n2 + 1, c = t % F.n2 + 1;
+      for (int a = 1; a <= F.n0; ++a) {
+        const int g = a * F.sx + b * F.sy + c;
+        s[F.ot + g] = s[F.orr + g] - ((P)6 * e[g] - (e[g - F.sx] + e[g + F.sx] + e[g - F.sy] + e[g + F.sy] +
+                                                      e[g - 1] + e[g + 1]));
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < C.n1 * C.n2; t += kTailThreads) {
+      const int b = t / C.n2 + 1, c = t % C.n2 + 1;
+      for (int a = 1; a <= C.n0; ++a) {
+        const int f = (2 * a) * F.sx + (2 * b) * F.sy + 2 * c;
+        P acc = 0;
+        for (int da = -1; da <= 1; ++da)
+          for (int db = -1; db <= 1; ++db)
+            for (int dc = -1; dc <= 1; ++dc)
+              acc += fw_weight<P>(da, db, dc) * s[F.ot + f + da * F.sx + db * F.sy + dc];
+        const int gc = a * C.sx + b * C.sy + c;
+        s[C.orr + gc] = acc * (P)0.5;
+        s[C.oe + gc] = (P)0;
+      }
+    }
+    __syncthreads();
+  }
+  // coarsest level
+  {
+    const TailLv& L = A.lv[A.nl - 1];
+    if (L.n0 == 1 && L.n1 == 1 && L.n2 == 1) {
+      if (threadIdx.x == 0) {
+        const int g = L.sx + L.sy + 1;
+        s[L.oe + g] = s[L.orr + g] / (P)6;
+      }
+      __syncthreads();
+    } else {
+      tail_smooth<P>(s, L, A.coarse_sweeps, (P)0, (P)1 / (P)6);
+    }
+  }
+  // ascend
+  for (int l = A.nl - 2; l >= 0; --l) {
+    const TailLv& F = A.lv[l];
+    const TailLv& C = A.lv[l + 1];
+    const P* ec = s + C.oe;
+    for (int t = threadIdx.x; t < F.n1 * F.n2; t += kTailThreads) {
+      const int b = t / F.n2 + 1, c = t % F.n2 + 1;
+      const int b0 = b >> 1, b1 = (b + 1) >> 1, c0 = c >> 1, c1 = (c + 1) >> 1;
+      for (int a = 1; a <= F.n0; ++a) {
+        const int a0 = a >> 1, a1 = (a + 1) >> 1;
+        auto E = [&](int x, int y, int z) { return ec[x * C.sx + y * C.sy + z]; };
+        const P v = (E(a0, b0, c0) + E(a1, b0, c0) + E(a0, b1, c0) + E(a1, b1, c0) + E(a0, b0, c1) +
+                     E(a1, b0, c1) + E(a0, b1, c1) + E(a1, b1, c1)) *
+                    (P)0.125;
+        s[F.oe + a * F.sx + b * F.sy + c] += v;
+      }
+    }
+    __syncthreads();
+    tail_smooth<P>(s, F, A.post, A.w1, A.w6);
+  }
+  {
+    const TailLv& L = A.lv[0];
+    for (int t = threadIdx.x; t < L.n1 * L.n2; t += kTailThreads) {
+      const int b = t / L.n2 + 1, c = t % L.n2 + 1;
+      for (int a = 1; a <= L.n0; ++a) A.ge[a * A.gsx + b * A.gsy + c] = s[L.oe + a * L.sx + b * L.sy + c];
+    }
+  }
 }
 
 // Face conditions on phi (Field.cpp:10-96): D -> 0, N -> copy the neighbour.
@@ -228,34 +453,100 @@ float* T_<float>(MgLevel& m) {
 dim3 grid_full(const MgLevel& m) { return dim3(div_up(m.n[2], 64), m.n[1], m.n[0]); }
 dim3 grid_half(const MgLevel& m) { return dim3(div_up((m.n[2] + 1) / 2, 64), m.n[1], m.n[0]); }
 
-template <class P>
-void smooth(Ctx& c, MgLevel& m, int sweeps, double w) {
-  for (int s = 0; s < sweeps; ++s)
-    for (int col = 0; col < 2; ++col)
-      k_smooth<P><<<grid_half(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), col, (P)w);
+int coarse_sweeps(const MgLevel& m) { return 4 * std::max({m.n[0], m.n[1], m.n[2]}) + 8; }
+
+// First level handled by the LDS tail kernel and its LDS bytes, for element size `esz`.
+void plan_tail(Ctx& c, size_t esz) {
+  const int nl = (int)c.mg.size();
+  c.mg_tail = nl;  // none
+  c.mg_tail_bytes = 0;
+  size_t bytes = 0;
+  for (int l = nl - 1; l >= 0; --l) {
+    const MgLevel& m = c.mg[l];
+    // the tail keeps its levels as padded (n+2)^3 arrays
+    const size_t lsz = (size_t)(m.n[0] + 2) * (m.n[1] + 2) * (m.n[2] + 2);
+    const size_t lb = 3 * lsz * esz;
+    if (bytes + lb > c.lds_max || nl - l > kTailMaxLevels) break;
+    bytes += lb;
+    c.mg_tail = l;
+    c.mg_tail_bytes = bytes;
+  }
 }
 
 template <class P>
-void vcycle(Ctx& c, int l) {
+TailArgs<P> tail_args(Ctx& c, const int* done) {
+  TailArgs<P> A{};
+  const int l0 = c.mg_tail;
+  A.nl = (int)c.mg.size() - l0;
+  int off = 0;
+  for (int q = 0; q < A.nl; ++q) {
+    const MgLevel& m = c.mg[l0 + q];
+    TailLv& L = A.lv[q];
+    L.n0 = m.n[0];
+    L.n1 = m.n[1];
+    L.n2 = m.n[2];
+    L.sy = m.n[2] + 2;
+    L.sx = (m.n[1] + 2) * (m.n[2] + 2);
+    L.size = (m.n[0] + 2) * L.sx;
+    L.oe = off;
+    L.orr = off + L.size;
+    L.ot = off + 2 * L.size;
+    off += 3 * L.size;
+  }
+  MgLevel& m0 = c.mg[l0];
+  A.ge = E_<P>(m0);
+  A.gr = R_<P>(m0);
+  A.gsx = m0.sx;
+  A.gsy = m0.sy;
+  A.w1 = (P)(1.0 - c.d.mg_omega);
+  A.w6 = (P)(c.d.mg_omega / 6.0);
+  A.pre = c.d.mg_pre;
+  A.post = c.d.mg_post;
+  A.coarse_sweeps = coarse_sweeps(c.mg.back());
+  A.total = off;
+  A.done = done;
+  return A;
+}
+
+template <class P>
+void smooth(Ctx& c, MgLevel& m, int sweeps, double w, bool from_zero, const int* done) {
+  const P w1 = (P)(1.0 - w), w6 = (P)(w / 6.0);
+  for (int s = 0; s < sweeps; ++s)
+    for (int col = 0; col < 2; ++col) {
+      if (from_zero && s == 0 && col == 0)
+        k_smooth_first<P><<<grid_full(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), w6, done);
+      else
+        k_smooth<P><<<grid_half(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), col, w1, w6, done);
+    }
+}
+
+template <class P>
+void vcycle(Ctx& c, int l, const int* done) {
   MgLevel& m = c.mg[l];
   const double w = c.d.mg_omega;
+  if (l == c.mg_tail) {
+    TailArgs<P> A = tail_args<P>(c, done);
+    k_coarse_tail<P><<<1, kTailThreads, c.mg_tail_bytes, c.stream>>>(A);
+    return;
+  }
   if (l == (int)c.mg.size() - 1) {
+    // coarsest level outside the tail kernel: Gauss-Seidel sweeps from zero
     if (m.n[0] == 1 && m.n[1] == 1 && m.n[2] == 1) {
-      k_coarse_single<P><<<1, 1, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m));
+      k_coarse_single<P><<<1, 1, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), done);
     } else {
-      const int sweeps = 4 * std::max({m.n[0], m.n[1], m.n[2]}) + 8;
-      smooth<P>(c, m, sweeps, 1.0);
+      smooth<P>(c, m, coarse_sweeps(m), 1.0, true, done);
     }
     return;
   }
   MgLevel& mc = c.mg[l + 1];
-  smooth<P>(c, m, c.d.mg_pre, w);
-  k_resid<P><<<grid_full(m), 64, 0, c.stream>>>(T_<P>(m), E_<P>(m), R_<P>(m), lv_of(m));
-  k_restrict<P><<<grid_full(mc), 64, 0, c.stream>>>(R_<P>(mc), lv_of(mc), T_<P>(m), lv_of(m));
-  k_zero<P><<<div_up((int64_t)mc.size, 256), 256, 0, c.stream>>>(E_<P>(mc) - mc.off, (int64_t)mc.size);
-  vcycle<P>(c, l + 1);
-  k_prolong<P><<<grid_full(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), E_<P>(mc), lv_of(mc));
-  smooth<P>(c, m, c.d.mg_post, w);
+  // every level starts its cycle from e = 0 (the correction equation)
+  if (c.d.mg_pre == 0) k_zero_level<P><<<grid_full(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), done);
+  smooth<P>(c, m, c.d.mg_pre, w, true, done);
+  k_resid<P><<<grid_full(m), 64, 0, c.stream>>>(T_<P>(m), E_<P>(m), R_<P>(m), lv_of(m), done);
+  k_restrict<P><<<grid_full(mc), 64, 0, c.stream>>>(R_<P>(mc), lv_of(mc), T_<P>(m), lv_of(m), done);
+  vcycle<P>(c, l + 1, done);
+  k_prolong<P><<<grid_full(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), E_<P>(mc), lv_of(mc), done);
+  smooth<P>(c, m, c.d.mg_post, w, false, done);
 }
 
 template <class P>
@@ -269,25 +560,42 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   P* r0 = R_<P>(m0);
   const double rc = 1.0 / c.d.eps0;  // pow(dx, 2) / eps0 with dx = 1
   const int nb = 512;
-  c.red.alloc(std::max<size_t>(c.red.n, 2 * nb + 2));
-  double nrm[2];
-  auto residual = [&]() {
-    k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, c.red.p);
-    k_sum2<<<1, 256, 0, c.stream>>>(c.red.p, nb, c.red.p + 2 * nb);
-    FOPIS_HIP(hipMemcpyAsync(nrm, c.red.p + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    FOPIS_HIP(hipStreamSynchronize(c.stream));
-  };
-  auto rel = [&]() { return nrm[1] > 0.0 ? std::sqrt(nrm[0] / nrm[1]) : std::sqrt(nrm[0]); };
-  residual();
-  int cycles = 0;
-  while (rel() > tol && cycles < max_cycles) {
-    if (nrm[1] == 0.0 && nrm[0] < 1e-300) break;
-    k_zero<P><<<div_up((int64_t)m0.size, 256), 256, 0, c.stream>>>(E_<P>(m0) - off, (int64_t)m0.size);
-    vcycle<P>(c, 0);
-    k_add<P><<<grid_full(m0), 64, 0, c.stream>>>(phi, E_<P>(m0), L0);
-    residual();
-    ++cycles;
+  c.red.alloc(std::max<size_t>(c.red.n, 2 * nb + 4));
+  double* part = c.red.p;
+  double* dst = c.red.p + 2 * nb;
+  int* st = c.mg_state.p;
+  const int* done = st;
+  if (c.mg_tail_esz != sizeof(P)) {
+    plan_tail(c, sizeof(P));
+    c.mg_tail_esz = sizeof(P);
+    if (c.mg_tail < (int)c.mg.size() && c.mg_tail_bytes > 65536)
+      FOPIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarse_tail<P>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.mg_tail_bytes));
   }
+
+  k_solve_begin<<<1, 1, 0, c.stream>>>(st);
+  k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, part, done);
+  k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, 0);
+  // queue as many cycles as the previous solves needed, then two at a time
+  int batch = std::max(1, std::min(c.mg_hist[0], c.mg_hist[1]));
+  int32_t hs[2];
+  double hd[3];
+  for (;;) {
+    for (int q = 0; q < batch; ++q) {
+      vcycle<P>(c, 0, done);
+      k_add<P><<<grid_full(m0), 64, 0, c.stream>>>(phi, E_<P>(m0), L0, done);
+      k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, part, done);
+      k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, 1);
+    }
+    FOPIS_HIP(hipMemcpyAsync(hs, st, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+    FOPIS_HIP(hipMemcpyAsync(hd, dst, sizeof(hd), hipMemcpyDeviceToHost, c.stream));
+    FOPIS_HIP(hipStreamSynchronize(c.stream));
+    if (hs[0]) break;
+    batch = 2;
+  }
+  const int cycles = hs[1];
+  c.mg_hist[1] = c.mg_hist[0];
+  c.mg_hist[0] = cycles;
   // face conditions in the reference's order: x low/up, y, z
   for (int axis = 0; axis < 3; ++axis)
     for (int side = 0; side < 2; ++side) {
@@ -307,7 +615,7 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
       k_face<<<dim3(div_up(nu, 64), nv), 64, 0, c.stream>>>(c.phi.p, c.cx, c.cy, c.cz, axis, side, bcc == 'N');
     }
   FOPIS_HIP(hipGetLastError());
-  *relres_out = rel();
+  *relres_out = hd[2];
   return cycles;
 }
 
@@ -353,6 +661,15 @@ void poisson_init(Ctx& c) {
     if (!can) break;
     for (int q = 0; q < 3; ++q) n[q] = (n[q] - 1) / 2;
   }
+  c.mg_state.alloc(4);
+  FOPIS_HIP(hipMemsetAsync(c.mg_state.p, 0, 4 * sizeof(int32_t), c.stream));
+  int dev = 0;
+  FOPIS_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t pr;
+  FOPIS_HIP(hipGetDeviceProperties(&pr, dev));
+  c.lds_max = pr.sharedMemPerBlock;
+  c.mg_tail_esz = 0;
+  c.mg_hist[0] = c.mg_hist[1] = 1;
   FOPIS_HIP(hipStreamSynchronize(c.stream));
 }
 

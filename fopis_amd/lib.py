@@ -27,6 +27,10 @@ EXPORTS = [
     "fopis_set_object", "fopis_build_cmatrix", "fopis_get_cmatrix", "fopis_set_cmatrix",
     "fopis_add_object_charge", "fopis_get_object_state", "fopis_set_object_charge_map", "fopis_phase_time",
     "fopis_reset_timers", "fopis_synchronize", "fopis_solver_stats", "fopis_load_maxwellian",
+    "fopis_guess_init", "fopis_guess_num_params", "fopis_guess_get_params", "fopis_guess_set_params",
+    "fopis_guess_set_mode", "fopis_guess_apply", "fopis_guess_dataset_alloc", "fopis_guess_dataset_store",
+    "fopis_guess_dataset_get", "fopis_guess_dataset_set", "fopis_guess_grad", "fopis_guess_train",
+    "fopis_guess_eval", "fopis_guess_synth_boxes", "fopis_nn_conv3d", "fopis_nn_conv3d_backward",
 ]
 
 

@@ -203,6 +203,70 @@ int fopis_solver_stats(fopis_ctx* ctx, int32_t* last_cycles, double* last_relres
 /* Blocks until the context's stream is idle. */
 int fopis_synchronize(fopis_ctx* ctx);
 
+/* ---- learned initial guess for the Poisson solve ---------------------------
+ *
+ * No counterpart in the reference, whose solve is SOR warm-started from the
+ * previous potential (RootGridField.cpp:41-103). A 3-D convolutional
+ * encoder-decoder (4 levels, 16-64 channels, 3^3 kernels, BF16 weights and
+ * activations, FP32 accumulation) maps (rho[0], conductor node mask) to phi at
+ * the interior nodes; the multigrid solve then iterates from it to the same
+ * residual as always. The network grid has (n-2) rounded up to 8 voxels per
+ * axis; voxel (i,j,k) is glue node (i+2, j+2, k+2).
+ * Parameters are one float vector: per layer W[27][cin][cout] then b[cout],
+ * layers e0a 8->16, e0b 16->16, e1a 16->32, e1b 32->32, e2a 32->48, e2b 48->48,
+ * e3a 48->64, e3b 64->64, d2a 64->48, d2b 48->48, d1a 48->32, d1b 32->32,
+ * d0a 32->16, d0b 16->16, out 16->16 (channel 0 used); tap (di,dj,dk) is
+ * (di+1)*9 + (dj+1)*3 + (dk+1). */
+
+/* Creates the network on the context's grid with a seeded He-normal start. */
+int fopis_guess_init(fopis_ctx* ctx, uint64_t seed);
+int fopis_guess_num_params(fopis_ctx* ctx, int64_t* n);
+/* Master (FP32) parameters and the output scale k: phi = out * k * rms(rho) / eps0. */
+int fopis_guess_get_params(fopis_ctx* ctx, float* params, double* out_scale);
+int fopis_guess_set_params(fopis_ctx* ctx, const float* params, double out_scale);
+/* 0: every solve starts from the previous phi (reference behaviour);
+ * 1: the first solve of each step starts from the network's guess. */
+int fopis_guess_set_mode(fopis_ctx* ctx, int mode);
+/* phi at the interior nodes <- network(rho[0], mask). */
+int fopis_guess_apply(fopis_ctx* ctx);
+/* Device dataset of training pairs: rho_hat = rho[0]/rms(rho[0]) (BF16), node
+ * mask (u8), phi_hat = phi*eps0/rms(rho[0]) (BF16), each network-grid sized. */
+int fopis_guess_dataset_alloc(fopis_ctx* ctx, int32_t capacity);
+/* Stores the context's current (rho[0], mask, phi) as pair `slot`. */
+int fopis_guess_dataset_store(fopis_ctx* ctx, int32_t slot, double* phi_hat_rms);
+int fopis_guess_dataset_get(fopis_ctx* ctx, int32_t slot, uint16_t* rho_hat, uint8_t* mask, uint16_t* phi_hat);
+int fopis_guess_dataset_set(fopis_ctx* ctx, int32_t slot, const uint16_t* rho_hat, const uint8_t* mask,
+                            const uint16_t* phi_hat);
+/* Loss 0.5*mean((out - phi_hat/k)^2) over interior voxels and its parameter
+ * gradient on pair `slot` (grad may be NULL); no update. */
+int fopis_guess_grad(fopis_ctx* ctx, int32_t slot, double* loss, float* grad);
+/* Adam over pairs [0, npairs), `epochs` shuffled passes, cosine learning rate
+ * from lr to lr/20; epoch_loss[e] = mean loss of pass e (may be NULL). */
+int fopis_guess_train(fopis_ctx* ctx, int32_t npairs, int32_t epochs, double lr, uint64_t seed,
+                      double* epoch_loss);
+int fopis_guess_eval(fopis_ctx* ctx, int32_t begin, int32_t end, double* loss);
+/* Training-data conductors: nbox axis-aligned boxes of glue cells
+ * lohi[6*b..6*b+5] = lo0, lo1, lo2, hi0, hi1, hi2 (half-open). rho[0] is cleared
+ * inside each box and its surface nodes get charge[b] times a smooth seeded
+ * modulation; the boxes become the mask the guess sees. nbox = 0 returns to
+ * the context's own conductor mask. */
+int fopis_guess_synth_boxes(fopis_ctx* ctx, int32_t nbox, const int32_t* lohi, const double* charge, uint64_t seed);
+
+/* One convolution of the network on host buffers (BF16 as uint16 bits):
+ * x [n0'][n1'][n2'][cin] (n/2 when up != 0: nearest upsampling on read),
+ * w [27][cin][cout] FP32 (rounded to BF16 on the device), b [cout],
+ * add [n0][n1][n2][cout] or NULL, y [n0][n1][n2][cout] = act(conv + b + add).
+ * reps > 0 also times reps launches into *ms (per launch). */
+int fopis_nn_conv3d(int32_t n0, int32_t n1, int32_t n2, int32_t cin, int32_t cout, int32_t up, int32_t relu,
+                    const uint16_t* x, const float* w, const float* b, const uint16_t* add, uint16_t* y, int32_t reps,
+                    float* ms);
+/* Gradients of that convolution for output gradient gy [n0][n1][n2][cout]:
+ * dw [27][cin][cout], db [cout] (FP32) and, if dx != NULL (cin multiple of 16),
+ * dx [n0][n1][n2][cin] at the output grid (sum it over 2x2x2 for up != 0). */
+int fopis_nn_conv3d_backward(int32_t n0, int32_t n1, int32_t n2, int32_t cin, int32_t cout, int32_t up,
+                             const uint16_t* x, const float* w, const uint16_t* gy, float* dw, float* db,
+                             uint16_t* dx);
+
 #ifdef __cplusplus
 }
 #endif
