@@ -509,9 +509,10 @@ int fopis_guess_grad(fopis_ctx* ctx, int32_t slot, double* loss, float* grad) {
   });
 }
 
-int fopis_guess_train(fopis_ctx* ctx, int32_t npairs, int32_t epochs, double lr, uint64_t seed, double* epoch_loss) {
+int fopis_guess_train(fopis_ctx* ctx, int32_t npairs, int32_t epochs, double lr, double lr_end, uint64_t seed,
+                      double* epoch_loss) {
   NEED_CTX
-  return guarded([&] { guess_train(ctx->c, npairs, epochs, lr, seed, epoch_loss); });
+  return guarded([&] { guess_train(ctx->c, npairs, epochs, lr, lr_end, seed, epoch_loss); });
 }
 
 int fopis_guess_eval(fopis_ctx* ctx, int32_t begin, int32_t end, double* loss) {

@@ -257,7 +257,7 @@ double guess_dataset_store(Ctx& c, int slot);
 void guess_dataset_get(Ctx& c, int slot, uint16_t* rho_hat, uint8_t* mask, uint16_t* phi_hat);
 void guess_dataset_set(Ctx& c, int slot, const uint16_t* rho_hat, const uint8_t* mask, const uint16_t* phi_hat);
 double guess_grad(Ctx& c, int slot, float* grad_out);
-void guess_train(Ctx& c, int nsamples, int epochs, double lr, uint64_t seed, double* loss_out);
+void guess_train(Ctx& c, int nsamples, int epochs, double lr, double lr_end, uint64_t seed, double* loss_out);
 double guess_eval(Ctx& c, int begin, int end);
 void guess_synth_boxes(Ctx& c, int nbox, const int32_t* lohi, const double* q, uint64_t seed);
 void guess_clear_boxes(Ctx& c);

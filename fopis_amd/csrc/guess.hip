@@ -1129,8 +1129,8 @@ double guess_grad(Ctx& c, int slot, float* grad_out) {
 }
 
 // Adam over pairs [0, nsamples) for `epochs` passes in shuffled order; the
-// learning rate follows a cosine from lr to lr/20. loss_out[e] = mean loss of epoch e.
-void guess_train(Ctx& c, int nsamples, int epochs, double lr, uint64_t seed, double* loss_out) {
+// learning rate follows a cosine from lr to lr_end. loss_out[e] = mean loss of epoch e.
+void guess_train(Ctx& c, int nsamples, int epochs, double lr, double lr_end, uint64_t seed, double* loss_out) {
   GuessNet& g = net(c);
   if (nsamples < 1 || nsamples > g.capacity) throw Error(FOPIS_E_ARG, "nsamples out of range of the dataset");
   if (epochs < 1) throw Error(FOPIS_E_ARG, "epochs must be >= 1");
@@ -1174,7 +1174,7 @@ void guess_train(Ctx& c, int nsamples, int epochs, double lr, uint64_t seed, dou
       backward(c, g);
       ++g.adam_t;
       const double frac = (double)step / std::max<int64_t>(1, total - 1);
-      const double lr_t = lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * frac)));
+      const double lr_t = lr_end + (lr - lr_end) * 0.5 * (1.0 + std::cos(M_PI * frac));
       const float b1 = 0.9f, b2 = 0.999f;
       const float c1 = 1.f - std::pow(b1, (float)g.adam_t), c2 = 1.f - std::pow(b2, (float)g.adam_t);
       k_adam<<<div_up(g.nparams, 256), 256, 0, c.stream>>>(g.master.p, g.grad.p, g.m1.p, g.m2.p, g.nparams,

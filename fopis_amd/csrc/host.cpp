@@ -407,6 +407,12 @@ class GpuRootGrid {
   void load_initial();
 
  public:
+  // the initial load again with another seed (device loader only)
+  void reload(uint64_t seed) {
+    if (!opt_.device_loader) throw std::invalid_argument("reload needs the device loader");
+    opt_.device_seed = seed;
+    load_initial();
+  }
   void generateInitial(int s, std::vector<double> (&a)[6]);
   std::array<double, 6> flux(int s) const;
 
@@ -1055,6 +1061,11 @@ int fopis_host_main_loop_es(fopis_host* h) {
 }
 
 int fopis_host_timestep(fopis_host* h) { return h ? h->g->timestep() : -1; }
+
+int fopis_host_reload(fopis_host* h, uint64_t seed) {
+  if (!h) return FOPIS_E_ARG;
+  return host_guard([&] { h->g->reload(seed); });
+}
 
 double fopis_host_generate_ms(fopis_host* h) { return h ? h->g->generate_ms() : 0.0; }
 

@@ -667,12 +667,35 @@ __global__ void k_keys_tile(SoaPtr<R> f, int64_t n, TileArgs<R> a, uint32_t* key
   keys[i] = key;
 }
 
-__global__ void k_count_tiles(const uint32_t* keys, int64_t n, int shift, uint32_t ntiles, uint32_t* cnt) {
+// Tile starts in a sorted key array: first[u] = index of the first key whose
+// tile (key >> shift, capped at ntiles for invalid particles) is >= u, for
+// u = 0..ntiles. Each thread writes the starts of the tiles that begin at its
+// index, so no two threads write one entry and no atomics are needed.
+__global__ void k_tile_starts(const uint32_t* keys, int64_t n, int shift, uint32_t ntiles, int64_t* first) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t t = keys[i] >> shift;
-  if (t >= ntiles) return;
-  atomicAdd(&cnt[t], 1u);
+  const uint32_t t = min(keys[i] >> shift, ntiles);
+  const uint32_t tp = i == 0 ? 0u : min(keys[i - 1] >> shift, ntiles) + 1u;
+  for (uint32_t u = tp; u <= t; ++u) first[u] = i;
+  if (i == n - 1)
+    for (uint32_t u = t + 1; u <= ntiles; ++u) first[u] = n;
+}
+
+__global__ void k_counts_from_starts(const int64_t* first, uint32_t ntiles, uint32_t* cnt) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u > ntiles) return;
+  cnt[u] = u < ntiles ? (uint32_t)(first[u + 1] - first[u]) : 0u;
+}
+
+// Per-tile counts of n sorted keys (cnt[ntiles] = 0); `first` is scratch of ntiles+1.
+void count_sorted_tiles(hipStream_t st, const uint32_t* keys, int64_t n, int shift, int nt, int64_t* first,
+                        uint32_t* cnt) {
+  if (n <= 0) {
+    FOPIS_HIP(hipMemsetAsync(cnt, 0, (nt + 1) * sizeof(uint32_t), st));
+    return;
+  }
+  k_tile_starts<<<div_up(n, 256), 256, 0, st>>>(keys, n, shift, (uint32_t)nt, first);
+  k_counts_from_starts<<<div_up(nt + 1, 256), 256, 0, st>>>(first, (uint32_t)nt, cnt);
 }
 
 __global__ void k_capacity(const uint32_t* cnt, int nt, double slack, int minc, uint32_t* capa, uint32_t* ocapa) {
@@ -955,8 +978,7 @@ void bin_flat(Ctx& c, int s, Soa<R>& flat, int64_t n, bool key_cell) {
                                                    (uint32_t)nt << shift);
     sc.sorter.sort(st, sc.keys.p, sc.perm.p, n, bits, true);
   }
-  FOPIS_HIP(hipMemsetAsync(sc.cnt.p, 0, (nt + 1) * sizeof(uint32_t), st));
-  if (n > 0) k_count_tiles<<<div_up(n, 256), 256, 0, st>>>(sc.keys.p, n, shift, (uint32_t)nt, sc.cnt.p);
+  count_sorted_tiles(st, sc.keys.p, n, shift, nt, sc.first.p, sc.cnt.p);
   exclusive_scan_u32(st, sc.cnt.p, nt + 1, sc.first.p, sc.tmp);
   if (!key_cell) {
     k_capacity<<<div_up(nt, 256), 256, 0, st>>>(sc.cnt.p, nt, c.d.capacity_slack, 64, sc.capa.p, sc.ocapa.p);
@@ -1066,8 +1088,7 @@ void prepare_inbox(Ctx& c, int s, bool consume) {
   sc.sorter.sort(st, sc.keys.p, sc.perm.p, n, bits, true);
   sc.cnt.alloc(std::max<size_t>(sc.cnt.n, nt + 1));
   sc.first.alloc(std::max<size_t>(sc.first.n, nt + 1));
-  FOPIS_HIP(hipMemsetAsync(sc.cnt.p, 0, (nt + 1) * sizeof(uint32_t), st));
-  k_count_tiles<<<div_up(n, 256), 256, 0, st>>>(sc.keys.p, n, 0, (uint32_t)nt, sc.cnt.p);
+  count_sorted_tiles(st, sc.keys.p, n, 0, nt, sc.first.p, sc.cnt.p);
   exclusive_scan_u32(st, sc.cnt.p, nt + 1, sc.first.p, sc.tmp);
   b.ibeg.alloc(nt + 1);
   b.inbox.alloc(n);

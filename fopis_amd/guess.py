@@ -67,7 +67,7 @@ def _bind(L):
     L.fopis_guess_dataset_get.argtypes = [vp, i32, u16p, u8p, u16p]
     L.fopis_guess_dataset_set.argtypes = [vp, i32, u16p, u8p, u16p]
     L.fopis_guess_grad.argtypes = [vp, i32, dp, fp]
-    L.fopis_guess_train.argtypes = [vp, i32, i32, d, u64, dp]
+    L.fopis_guess_train.argtypes = [vp, i32, i32, d, d, u64, dp]
     L.fopis_guess_eval.argtypes = [vp, i32, i32, dp]
     L.fopis_guess_synth_boxes.argtypes = [vp, i32, i32p, dp, u64]
     L.fopis_nn_conv3d.argtypes = [i32] * 7 + [u16p, fp, fp, u16p, u16p, i32, fp]
@@ -149,9 +149,10 @@ class Guess:
         _check(self.L, self.L.fopis_guess_grad(self.ctx.h, slot, C.byref(loss), _p(g, C.c_float)))
         return loss.value, g
 
-    def train(self, npairs: int, epochs: int, lr: float = 1e-3, seed: int = 7):
+    def train(self, npairs: int, epochs: int, lr: float = 1e-3, lr_end: float | None = None, seed: int = 7):
         out = np.zeros(epochs)
-        _check(self.L, self.L.fopis_guess_train(self.ctx.h, npairs, epochs, lr, seed, _p(out, C.c_double)))
+        lr_end = lr / 20 if lr_end is None else lr_end
+        _check(self.L, self.L.fopis_guess_train(self.ctx.h, npairs, epochs, lr, lr_end, seed, _p(out, C.c_double)))
         return out
 
     def eval(self, begin: int, end: int) -> float:

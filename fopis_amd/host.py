@@ -41,6 +41,7 @@ def _bind(L):
     L.fopis_host_desc.argtypes = [vp, C.POINTER(F.Desc)]
     L.fopis_host_main_loop_es.argtypes = [vp]
     L.fopis_host_timestep.argtypes = [vp]
+    L.fopis_host_reload.argtypes = [vp, C.c_uint64]
     L.fopis_host_generate_ms.argtypes = [vp]
     L.fopis_host_generate_ms.restype = C.c_double
     L.fopis_host_generate_injection.argtypes = [vp, C.c_int, i64] + [dp] * 6
@@ -124,6 +125,10 @@ class Host:
 
     def timestep(self) -> int:
         return self.L.fopis_host_timestep(self.h)
+
+    def reload(self, seed: int) -> None:
+        """Redraw the initial particle load on the device with another seed."""
+        self._ok(self.L.fopis_host_reload(self.h, seed))
 
     def generate_ms(self) -> float:
         return self.L.fopis_host_generate_ms(self.h)

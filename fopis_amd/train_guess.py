@@ -1,0 +1,189 @@
+"""Train the learned Poisson initial guess on pairs the solver generates.
+
+Recipe (the 128^3 learned-guess workload): rho is the deposit of a freshly
+drawn plasma of the given configuration (its species, densities and
+temperatures, drawn on the device) plus 1-3 random axis-aligned conductor
+boxes of 8-32 cells per side whose surface nodes carry a seeded charge;
+phi is the multigrid solution of that rho to relative residual 1e-8 from
+zero. Pairs [0, ntrain) train the network, pairs after that are held out.
+Training, inference and the pair store all run in libfopis_hip.so.
+
+Writes <out>.npz (params float32, out_scale, grid, and the recipe) and prints
+one JSON line with the losses and, on held-out problems, multigrid cycles and
+milliseconds from zero and from the network's guess.
+
+  python -m fopis_amd.train_guess --config fopis_amd/configs/c2.json \
+      --ntrain 2000 --nheld 200 --epochs 8 --out fopis_amd/weights/guess_c4
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+from fopis_amd import lib as F  # noqa: E402
+from fopis_amd.guess import Guess  # noqa: E402
+from fopis_amd.host import Host  # noqa: E402
+
+
+def log(msg):
+    print(f"[train_guess] {msg}", file=sys.stderr, flush=True)
+
+
+def plasma_config(path: str) -> str:
+    """The configuration without its objects (the boxes stand in for them)."""
+    with open(path) as f:
+        cfg = json.load(f)
+    cfg["Object"] = {}
+    fd, out = tempfile.mkstemp(suffix=".json")
+    with os.fdopen(fd, "w") as f:
+        json.dump(cfg, f)
+    return out
+
+
+def draw_problem(rng, nx: int, rho_rms: float, smin: int, smax: int):
+    """1-3 boxes (glue cell ranges) and their surface charge per node."""
+    boxes, charges = [], []
+    for _ in range(int(rng.integers(1, 4))):
+        s = rng.integers(smin, smax + 1, 3)
+        lo = [int(rng.integers(2, nx - 1 - int(s[a]))) for a in range(3)]
+        boxes.append(lo + [lo[a] + int(s[a]) for a in range(3)])
+        charges.append(float(rng.choice([-1.0, 1.0]) * rho_rms * 10.0 ** rng.uniform(-0.5, 1.0)))
+    return boxes, charges
+
+
+class PairMaker:
+    def __init__(self, host: Host, guess: Guess, seed: int, smin: int, smax: int):
+        self.h, self.g, self.ctx = host, guess, host.ctx
+        self.seed, self.smin, self.smax = seed, smin, smax
+        self.zero = np.zeros(self.ctx.field_shape(F.PHI))
+        self.h.reload(seed)
+        self.ctx.update_rho()
+        rho = self.ctx.get_field(F.RHO)
+        n = self.ctx.nx
+        self.rho_rms = float(np.sqrt(np.mean(rho[2:n, 2:n, 2:n] ** 2)))
+
+    def setup(self, i: int):
+        """rho and the boxes of problem i (deterministic in i); phi = 0."""
+        rng = np.random.default_rng(self.seed * 1_000_003 + i)
+        self.h.reload(self.seed + 7 * i + 1)
+        self.ctx.update_rho()
+        boxes, charges = draw_problem(rng, self.ctx.nx, self.rho_rms, self.smin, self.smax)
+        self.g.synth_boxes(boxes, charges, seed=self.seed + i)
+        self.ctx.set_field(F.PHI, self.zero)
+
+    def solve(self):
+        self.ctx.reset_timers()
+        cyc, rr = self.ctx.solve_poisson()
+        ms = self.ctx.phase_time("poisson")[0]
+        return cyc, rr, ms
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default=os.path.join(HERE, "configs", "c2.json"))
+    ap.add_argument("--ntrain", type=int, default=2000)
+    ap.add_argument("--nheld", type=int, default=200)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr-end", type=float, default=1e-4)
+    ap.add_argument("--seed", type=int, default=20251202)
+    ap.add_argument("--box-min", type=int, default=8)
+    ap.add_argument("--box-max", type=int, default=32)
+    ap.add_argument("--eval-problems", type=int, default=16, help="held-out problems for the cycle comparison")
+    ap.add_argument("--init", default="", help="start from these weights (.npz)")
+    ap.add_argument("--out", default="")
+    args = ap.parse_args(argv)
+
+    cfg = plasma_config(args.config)
+    t0 = time.perf_counter()
+    h = Host(cfg, device_loader=True, device_seed=args.seed, build_cmatrix=False, initialize=False)
+    os.unlink(cfg)
+    ctx = h.ctx
+    g = Guess(ctx, seed=args.seed)
+    if args.init:
+        z = np.load(args.init, allow_pickle=False)
+        g.set_params(z["params"], float(z["out_scale"]))
+    ntot = args.ntrain + args.nheld
+    g.dataset_alloc(ntot)
+    pm = PairMaker(h, g, args.seed, args.box_min, args.box_max)
+    log(f"grid {g.grid}, {g.nparams} parameters, rho rms {pm.rho_rms:.4g}, setup {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    cycles = []
+    for i in range(ntot):
+        pm.setup(i)
+        cyc, rr, _ = pm.solve()
+        if not rr <= 1e-8:
+            raise RuntimeError(f"pair {i}: solve stopped at relres {rr}")
+        cycles.append(cyc)
+        g.dataset_store(i)
+        if (i + 1) % max(1, ntot // 10) == 0:
+            log(f"pairs {i + 1}/{ntot} ({time.perf_counter() - t0:.1f} s)")
+    gen_s = time.perf_counter() - t0
+
+    held = (args.ntrain, ntot) if args.nheld else (0, args.ntrain)
+    held_before = g.eval(*held)
+    t0 = time.perf_counter()
+    losses = []
+    for e in range(args.epochs):
+        # one cosine over all epochs, taken one epoch per call
+        f0, f1 = e / args.epochs, (e + 1) / args.epochs
+        lr0 = args.lr_end + (args.lr - args.lr_end) * 0.5 * (1 + math.cos(math.pi * f0))
+        lr1 = args.lr_end + (args.lr - args.lr_end) * 0.5 * (1 + math.cos(math.pi * f1))
+        losses.append(float(g.train(args.ntrain, 1, lr0, lr1, seed=args.seed + e)[0]))
+        log(f"epoch {e + 1}/{args.epochs}: train loss {losses[-1]:.5g} ({time.perf_counter() - t0:.1f} s)")
+    train_s = time.perf_counter() - t0
+    held_after = g.eval(*held)
+    params, k = g.get_params()
+
+    # held-out problems: multigrid from zero vs from the network
+    rz, rg = [], []
+    for j in range(min(args.eval_problems, max(1, args.nheld))):
+        i = args.ntrain + j
+        pm.setup(i)
+        rz.append(pm.solve())
+        pm.setup(i)
+        ctx.reset_timers()
+        g.apply()
+        gms = ctx.phase_time("guess")[0]
+        cyc, rr, ms = pm.solve()
+        rg.append((cyc, rr, ms, gms))
+    res = {
+        "grid": list(g.grid), "params": int(g.nparams), "out_scale": k,
+        "pairs": {"train": args.ntrain, "held_out": args.nheld, "generate_s": gen_s,
+                  "cycles_from_zero_mean": float(np.mean(cycles))},
+        "loss": {"train_first_epoch": losses[0] if losses else None, "train_last_epoch": losses[-1] if losses else None,
+                 "held_out_before": held_before, "held_out_after": held_after},
+        "train_s": train_s, "steps_per_s": args.epochs * args.ntrain / max(train_s, 1e-9),
+        "eval": {
+            "problems": len(rz),
+            "cycles_from_zero": float(np.mean([r[0] for r in rz])),
+            "ms_from_zero": float(np.mean([r[2] for r in rz])),
+            "cycles_from_guess": float(np.mean([r[0] for r in rg])),
+            "ms_from_guess": float(np.mean([r[2] for r in rg])),
+            "guess_ms": float(np.mean([r[3] for r in rg])),
+        },
+    }
+    if args.out:
+        os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+        np.savez(args.out, params=params, out_scale=np.float64(k), grid=np.array(g.grid, np.int32),
+                 recipe=np.array(json.dumps({"config": os.path.basename(args.config), "ntrain": args.ntrain,
+                                             "epochs": args.epochs, "seed": args.seed,
+                                             "boxes": [args.box_min, args.box_max]})))
+        res["weights"] = args.out + ".npz"
+    h.close()
+    print(json.dumps(res), flush=True)
+    return res
+
+
+if __name__ == "__main__":
+    main()

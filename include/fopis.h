@@ -240,9 +240,10 @@ int fopis_guess_dataset_set(fopis_ctx* ctx, int32_t slot, const uint16_t* rho_ha
 /* Loss 0.5*mean((out - phi_hat/k)^2) over interior voxels and its parameter
  * gradient on pair `slot` (grad may be NULL); no update. */
 int fopis_guess_grad(fopis_ctx* ctx, int32_t slot, double* loss, float* grad);
-/* Adam over pairs [0, npairs), `epochs` shuffled passes, cosine learning rate
- * from lr to lr/20; epoch_loss[e] = mean loss of pass e (may be NULL). */
-int fopis_guess_train(fopis_ctx* ctx, int32_t npairs, int32_t epochs, double lr, uint64_t seed,
+/* Adam over pairs [0, npairs), `epochs` shuffled passes, learning rate on a
+ * cosine from lr to lr_end over the call; epoch_loss[e] = mean loss of pass e
+ * (may be NULL). Adam's moments persist across calls. */
+int fopis_guess_train(fopis_ctx* ctx, int32_t npairs, int32_t epochs, double lr, double lr_end, uint64_t seed,
                       double* epoch_loss);
 int fopis_guess_eval(fopis_ctx* ctx, int32_t begin, int32_t end, double* loss);
 /* Training-data conductors: nbox axis-aligned boxes of glue cells

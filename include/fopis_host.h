@@ -49,6 +49,9 @@ int fopis_host_desc(fopis_host* h, fopis_desc* out);
 /* One RootGrid::mainLoopES: emission and injection on the host, the step on the device. */
 int fopis_host_main_loop_es(fopis_host* h);
 int fopis_host_timestep(fopis_host* h);
+/* Redraws every ambient species' initial load on the device with a new seed
+ * (the RootGrid constructor's load, RootGrid.cpp:57-79; device loader only). */
+int fopis_host_reload(fopis_host* h, uint64_t seed);
 /* Particles the next injection call would add, without staging them
  * (for parity of the host generator): returns the count, fills up to cap. */
 int64_t fopis_host_generate_injection(fopis_host* h, int s, int64_t cap, double* x, double* y, double* z, double* vx,
