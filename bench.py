@@ -31,6 +31,9 @@ WORKLOADS = {
            "name": "C2: 128^3 cells, 64 ppc x 2 species, floating 16^3 cube, x-face injection, FP64"},
     "c3": {"json": "c3.json", "bytes_per_particle": {0: 96, 1: 48},
            "name": "C3: 256^3 cells, 64 ppc x 2 species, floating 32^3 cube, B0=0.2 z, FP32 push/deposit"},
+    "c5": {"json": "c5.json", "bytes_per_particle": {0: 96, 1: 48},
+           "name": "C5: 256^3 cells, 64 ppc x 2 species + photoelectrons (8 per lit surface node per step, vth 0.3) "
+                   "from the -x face of an L-shaped floating spacecraft (48x16x16 + 16x48x16), B0=0.2 z, FP32"},
     "c1": {"json": "c1.json", "bytes_per_particle": {0: 96, 1: 48},
            "name": "C1: 32^3 cells, 20 ppc x 2 species, no object, FP64"},
 }
@@ -66,6 +69,39 @@ def cpu_baseline(cfg_name: str, threads: int, ppc: int = 4, steps: int = 2) -> d
                       f"conductor, {ppc} ppc per species ({n} particles), best of {steps} steps of {best:.2f} s"}
 
 
+def learned_leg(h, steps: int, weights: str) -> dict:
+    """The first Poisson solve of each step from the network's guess (guess
+    inference + multigrid to the same tolerance) beside the same solve warm-
+    started from the previous step's phi, both over `steps` steps."""
+    import numpy as np
+
+    from fopis_amd import lib as F
+    from fopis_amd.guess import Guess
+
+    ctx = h.ctx
+    out = {"weights": os.path.relpath(weights, ROOT)}
+    z = np.load(weights, allow_pickle=False)
+    g = Guess(ctx, seed=1)
+    if tuple(int(v) for v in z["grid"]) != tuple(g.grid):
+        return {"skipped": f"weights are for grid {tuple(z['grid'])}, this grid is {g.grid}"}
+    g.set_params(z["params"], float(z["out_scale"]))
+    for mode, key in ((0, "plain"), (1, "learned")):
+        g.set_mode(mode)
+        h.main_loop_es()
+        ctx.synchronize()
+        ctx.reset_timers()
+        for _ in range(steps):
+            h.main_loop_es()
+        ctx.synchronize()
+        st = F.solver_stats(ctx)
+        out[key] = {"first_solve_ms": ctx.phase_time("poisson_first")[0],
+                    "first_solve_cycles": st["first_cycles"] / steps,
+                    "guess_ms": ctx.phase_time("guess")[0] if mode else 0.0}
+    g.set_mode(0)
+    out["speedup"] = out["plain"]["first_solve_ms"] / max(out["learned"]["first_solve_ms"], 1e-9)
+    return out
+
+
 def pmc_traffic(workload: str):
     p = os.path.join(ROOT, "profiles", f"tile_kernel_{workload}_traffic.json")
     if os.path.exists(p):
@@ -83,6 +119,9 @@ def main():
     ap.add_argument("--ppc", type=int, default=0, help="override particles per cell (0 = config)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cmatrix-cache", default="")
+    ap.add_argument("--guess-weights", default=os.path.join(ROOT, "fopis_amd", "weights", "guess_c4.npz"),
+                    help="trained learned-guess weights for the Poisson comparison (skipped if absent)")
+    ap.add_argument("--guess-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -135,6 +174,7 @@ def main():
     red_ms, _ = ctx.phase_time("reduce")
     poi_ms, n_poi = ctx.phase_time("poisson")
     ef_ms, _ = ctx.phase_time("efield")
+    first_ms, _ = ctx.phase_time("poisson_first")
     stats = F.solver_stats(ctx)
     n_avg = 0.5 * (nparts + n_end)
     value = n_avg / (part_ms * 1e-3)
@@ -162,6 +202,8 @@ def main():
         "poisson_ms": poi_ms,
         "poisson_solves_per_step": n_poi / max(1, n_part),
         "poisson_cycles_per_solve": stats["total_cycles"] / max(1, stats["total_solves"]),
+        "poisson_first_solve_ms": first_ms,
+        "poisson_first_solve_cycles": stats["first_cycles"] / max(1, args.steps),
         "efield_ms": ef_ms,
         "host_generate_ms_per_step": gen_ms / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -174,6 +216,11 @@ def main():
         # HBM bytes of one k_tile launch (one species) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
         line["roofline"]["traffic"] = tr.get("bytes_per_launch")
         line["roofline"]["algorithmic_bytes_per_launch"] = bpp * n_avg / h.S
+    if args.guess_steps > 0 and os.path.exists(args.guess_weights):
+        try:
+            line["poisson_learned_vs_plain"] = learned_leg(h, args.guess_steps, args.guess_weights)
+        except Exception as e:  # reported, never silently replaced
+            line["poisson_learned_vs_plain"] = {"error": str(e)}
     h.close()
     if not args.no_cpu_baseline:
         try:

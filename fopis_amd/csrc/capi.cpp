@@ -71,8 +71,10 @@ static void update_rho(Ctx& c, bool push) {
   objects_sum_whole_charge(c);
   if (c.nobj > 0) {
     double rr = 0.0;
+    Phase ph(c, "poisson_first");  // guess (when on) + the step's first solve
     if (guess_mode(c) == 1) guess_apply(c);
     poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+    c.mg_first_cycles_total += c.mg_cycles_last;
     objects_redistribute(c);
   }
 }
@@ -302,8 +304,14 @@ int fopis_main_loop_es(fopis_ctx* ctx) {
     Ctx& c = ctx->c;
     update_rho(c, true);
     double rr = 0.0;
-    if (c.nobj == 0 && guess_mode(c) == 1) guess_apply(c);
-    poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+    if (c.nobj == 0) {
+      Phase ph(c, "poisson_first");
+      if (guess_mode(c) == 1) guess_apply(c);
+      poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+      c.mg_first_cycles_total += c.mg_cycles_last;
+    } else {
+      poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+    }
     if (!(rr <= c.d.poisson_tol)) throw Error(FOPIS_E_NOCONV, "Poisson solve stopped at relres " + std::to_string(rr));
     fields_efield(c);
     ++c.timestep;
@@ -416,6 +424,7 @@ int fopis_reset_timers(fopis_ctx* ctx) {
     ctx->c.timers.clear();
     ctx->c.mg_cycles_total = 0;
     ctx->c.mg_solves_total = 0;
+    ctx->c.mg_first_cycles_total = 0;
   });
 }
 
@@ -429,6 +438,24 @@ int fopis_solver_stats(fopis_ctx* ctx, int32_t* last_cycles, double* last_relres
     if (total_cycles) *total_cycles = c.mg_cycles_total;
     if (total_solves) *total_solves = c.mg_solves_total;
   });
+}
+
+int fopis_moments(fopis_ctx* ctx, int s, double* out) {
+  NEED_CTX
+  if (!out || s < 0 || s >= ctx->c.S) {
+    g_last_error = "bad species or null output";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { particles_moments(ctx->c, s, out); });
+}
+
+int fopis_first_solve_cycles(fopis_ctx* ctx, int64_t* total_cycles) {
+  NEED_CTX
+  if (!total_cycles) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { *total_cycles = ctx->c.mg_first_cycles_total; });
 }
 
 int fopis_guess_init(fopis_ctx* ctx, uint64_t seed) {

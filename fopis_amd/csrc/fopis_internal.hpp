@@ -192,8 +192,10 @@ struct Ctx {
   std::vector<MgLevel> mg;
   int mg_cycles_last = 0;
   int64_t mg_cycles_total = 0, mg_solves_total = 0;
+  int64_t mg_first_cycles_total = 0;  // cycles of the first solve of each step
   double relres_last = 0;
   DBuf<int32_t> mg_state;     // [4] solve flag and cycle count on the device
+  DBuf<double> mg_r64;        // FP64 level-0 residual when the V-cycle is FP32
   int mg_hist[2] = {1, 1};    // cycles of the last two solves (batch size of the next)
   int mg_tail = 0;            // first level solved by the single-workgroup LDS kernel
   size_t mg_tail_bytes = 0;   // its LDS bytes
@@ -223,6 +225,7 @@ void particles_deposit_bin(Ctx& c, bool push);
 void particles_sort_by_cell(Ctx& c);
 void particles_cell_keys(Ctx& c, int s, int64_t cap, int32_t* keys, int64_t* n);
 double particles_energy(Ctx& c);
+void particles_moments(Ctx& c, int s, double* out);
 
 void fields_init(Ctx& c);
 void fields_efield(Ctx& c);

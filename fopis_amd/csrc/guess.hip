@@ -83,26 +83,51 @@ struct ConvArgs {
   int relu;
 };
 
-template <int MT, int NT, int EPI, bool UP>
+// Output bricks of kBI x kBJ x 16 voxels per workgroup: the brick's input
+// with a one-voxel halo (kBI+2, kBJ+2, 18) is staged once in LDS, channel rows
+// padded by 8 elements so that the 16 lanes of a fragment read hit distinct
+// banks; each of the 4 waves then owns kBI*kBJ/4 rows of 16 voxels (its M
+// tiles) and reads its A fragments from LDS, B fragments (weights) from
+// global memory through the caches.
+constexpr int kBI = 4, kBJ = 4, kBK = 16;
+constexpr int kHI = kBI + 2, kHJ = kBJ + 2, kHK = kBK + 2;
+constexpr int kHalo = kHI * kHJ * kHK;
+constexpr int kRowsPerWave = kBI * kBJ / 4;
+
+inline size_t conv_lds_bytes(int cin) { return (size_t)kHalo * (cin + 8) * sizeof(uint16_t); }
+
+template <int NT, int EPI, bool UP>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t xs[];
+  constexpr int MT = kRowsPerWave;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int row = lane & 15, grp = lane >> 4;
-  const int64_t nvox = (int64_t)A.n0 * A.n1 * A.n2;
+  const int nb1 = (A.n1 + kBJ - 1) / kBJ, nb2 = (A.n2 + kBK - 1) / kBK;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t vw = ((int64_t)blk * 4 + wave) * (MT * 16);
-  if (vw >= nvox) return;
+  const int b2 = blk % nb2, b1 = (blk / nb2) % nb1, b0 = blk / (nb2 * nb1);
+  const int i0 = b0 * kBI, j0 = b1 * kBJ, k0 = b2 * kBK;
   const int cpt = A.cin >> 3;
+  const int ls = A.cin + 8;  // LDS elements per staged voxel
   const int in1 = UP ? (A.n1 >> 1) : A.n1, in2 = UP ? (A.n2 >> 1) : A.n2;
-  int vi[MT], vj[MT], vk[MT];
-  bool vin[MT];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    const int64_t v = vw + mt * 16 + row;
-    vin[mt] = v < nvox;
-    vk[mt] = (int)(v % A.n2);
-    vj[mt] = (int)((v / A.n2) % A.n1);
-    vi[mt] = (int)(v / ((int64_t)A.n1 * A.n2));
+
+  // stage the input brick with its halo (zeros outside the grid)
+  for (int e = threadIdx.x; e < kHalo * cpt; e += 256) {
+    const int ch = e % cpt, hv = e / cpt;
+    const int hk = hv % kHK, hj = (hv / kHK) % kHJ, hi = hv / (kHK * kHJ);
+    int gi = i0 - 1 + hi, gj = j0 - 1 + hj, gk = k0 - 1 + hk;
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if (gi >= 0 && gi < A.n0 && gj >= 0 && gj < A.n1 && gk >= 0 && gk < A.n2) {
+      if (UP) {
+        gi >>= 1;
+        gj >>= 1;
+        gk >>= 1;
+      }
+      v = *reinterpret_cast<const u32x4*>(A.x + (((size_t)gi * in1 + gj) * in2 + gk) * A.cin + ch * 8);
+    }
+    *reinterpret_cast<u32x4*>(xs + hv * ls + ch * 8) = v;
   }
+  __syncthreads();
+
   f32x4 acc[MT][NT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
@@ -115,33 +140,28 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
     bf16x8 bfr[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) bfr[nt] = load8(A.w + ((size_t)q * A.cout + nt * 16 + row) * 8);
-    const int tap = q / cpt;
-    const int c0 = (q - tap * cpt) * 8;
+    // chunks past the last tap (K padding) read chunk 0 of tap 0 and are zeroed
     const bool qok = q < A.nq;
-    const int di = tap / 9 - 1, dj = (tap / 3) % 3 - 1, dk = tap % 3 - 1;
+    const int tap = qok ? q / cpt : 0;
+    const int c0 = qok ? (q - tap * cpt) * 8 : 0;
+    const int di = tap / 9, dj = (tap / 3) % 3, dk = tap % 3;  // halo-relative offsets 0..2
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
-      int ii = vi[mt] + di, jj = vj[mt] + dj, kk = vk[mt] + dk;
-      const bool ok = qok && vin[mt] && ii >= 0 && ii < A.n0 && jj >= 0 && jj < A.n1 && kk >= 0 && kk < A.n2;
-      bf16x8 af = zero8();
-      if (ok) {
-        if (UP) {
-          ii >>= 1;
-          jj >>= 1;
-          kk >>= 1;
-        }
-        const size_t src = ((size_t)ii * in1 + jj) * in2 + kk;
-        af = load8(A.x + src * A.cin + c0);
-      }
+      const int r = wave * MT + mt;
+      const int hv = ((r / kBJ + di) * kHJ + (r % kBJ + dj)) * kHK + row + dk;
+      bf16x8 af = qok ? __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(xs + hv * ls + c0)) : zero8();
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
         acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[nt], acc[mt][nt], 0, 0, 0);
     }
   }
 
-  // D layout: lane holds output channel nt*16 + row for voxels 4*grp + r of the tile
+  // D layout: lane holds output channel nt*16 + row for voxels k0 + 4*grp + r of the row
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
+  for (int mt = 0; mt < MT; ++mt) {
+    const int rr = wave * MT + mt;
+    const int i = i0 + rr / kBJ, j = j0 + rr % kBJ;
+    if (i >= A.n0 || j >= A.n1) continue;
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
       const int co = nt * 16 + row;
@@ -149,8 +169,9 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
       const float bco = A.bias[co];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t v = vw + mt * 16 + grp * 4 + r;
-        if (v >= nvox) continue;
+        const int k = k0 + 4 * grp + r;
+        if (k >= A.n2) continue;
+        const int64_t v = ((int64_t)i * A.n1 + j) * A.n2 + k;
         float val = acc[mt][nt][r] + bco;
         if (EPI == EPI_BF16) {
           if (A.add) val += bf2f(A.add[v * A.cout + co]);
@@ -158,13 +179,12 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
           A.y[v * A.cout + co] = f2bf(val);
         } else if (EPI == EPI_F32) {
           A.y32[v] = val;
-        } else {
-          const int k = (int)(v % A.n2), j = (int)((v / A.n2) % A.n1), i = (int)(v / ((int64_t)A.n1 * A.n2));
-          if (i < A.vn0 && j < A.vn1 && k < A.vn2)
-            A.phi[A.goff + i * A.gsx + j * A.gsy + k] = (double)val * A.scal[0] * A.phi_k;
+        } else if (i < A.vn0 && j < A.vn1 && k < A.vn2) {
+          A.phi[A.goff + i * A.gsx + j * A.gsy + k] = (double)val * A.scal[0] * A.phi_k;
         }
       }
     }
+  }
 }
 
 // ---------------------------------------------------------------- weight gradient
@@ -417,14 +437,23 @@ __global__ __launch_bounds__(256) void k_rho2(const double* rho, Geo G, double* 
 }
 
 // scal[0] = rms of rho over interior nodes (1 when rho vanishes), scal[1] = 1/scal[0].
-__global__ void k_rho_scale(const double* part, int nb, int64_t n, double* scal) {
-  if (threadIdx.x != 0) return;
+// One block of 256 threads, fixed-order tree.
+__global__ __launch_bounds__(256) void k_rho_scale(const double* part, int nb, int64_t n, double* scal) {
+  __shared__ double sh[256];
   double t = 0.0;
-  for (int b = 0; b < nb; ++b) t += part[b];
-  double s = sqrt(t / (double)n);
-  if (!(s > 0.0)) s = 1.0;
-  scal[0] = s;
-  scal[1] = 1.0 / s;
+  for (int b = threadIdx.x; b < nb; b += 256) t += part[b];
+  sh[threadIdx.x] = t;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double s = sqrt(sh[0] / (double)n);
+    if (!(s > 0.0)) s = 1.0;
+    scal[0] = s;
+    scal[1] = 1.0 / s;
+  }
 }
 
 // Node mask: 1 if any of the 8 cells around the node belongs to a conductor.
@@ -639,9 +668,15 @@ void pack_all(Ctx& c, GuessNet& g) {
 
 template <int NT, int EPI, bool UP>
 void launch_conv_t(hipStream_t st, const ConvArgs& a) {
-  constexpr int MT = 2;
-  const int64_t nvox = (int64_t)a.n0 * a.n1 * a.n2;
-  k_conv<MT, NT, EPI, UP><<<div_up(nvox, 64 * MT), 256, 0, st>>>(a);
+  const int64_t nb = (int64_t)div_up(a.n0, kBI) * div_up(a.n1, kBJ) * div_up(a.n2, kBK);
+  const size_t lds = conv_lds_bytes(a.cin);
+  static size_t granted = 65536;  // dynamic LDS this instance may use (raised once as needed)
+  if (lds > granted) {
+    FOPIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv<NT, EPI, UP>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds_bytes(64)));
+    granted = conv_lds_bytes(64);
+  }
+  k_conv<NT, EPI, UP><<<(unsigned)nb, 256, lds, st>>>(a);
 }
 
 template <int EPI, bool UP>
@@ -903,9 +938,9 @@ double loss_and_grad(Ctx& c, GuessNet& g, int slot, bool sync) {
 }
 
 void compute_scale(Ctx& c, GuessNet& g, const double* rho) {
-  const int nb = 256;
+  const int nb = 1024;
   k_rho2<<<nb, 256, 0, c.stream>>>(rho, g.G, g.dpart.p);
-  k_rho_scale<<<1, 1, 0, c.stream>>>(g.dpart.p, nb, (int64_t)g.G.vn0 * g.G.vn1 * g.G.vn2, g.scal.p);
+  k_rho_scale<<<1, 256, 0, c.stream>>>(g.dpart.p, nb, (int64_t)g.G.vn0 * g.G.vn1 * g.G.vn2, g.scal.p);
 }
 
 }  // namespace

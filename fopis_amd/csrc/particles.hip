@@ -796,6 +796,35 @@ __global__ __launch_bounds__(kBlock) void k_v2(SoaPtr<R> f, int64_t n, double* p
   if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// Per-tile sums of x, y, z, vx, vy, vz and vx^2, vy^2, vz^2 (FP64) over the
+// tile's region in place (one workgroup per tile).
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_moments(SoaPtr<R> f, const int64_t* start, const int32_t* cnt,
+                                                    double* part) {
+  __shared__ double s[9][kBlock];
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int64_t b0 = start[blockIdx.x], n = b0 + cnt[blockIdx.x];
+  for (int64_t i = b0 + threadIdx.x; i < n; i += kBlock) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] += (double)f.c[q][i];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double v = (double)f.c[3 + q][i];
+      acc[6 + q] += v * v;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) s[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) part[(size_t)blockIdx.x * 9 + threadIdx.x] = s[threadIdx.x][0];
+}
+
 __global__ void k_sum_serial(const double* part, int n, double* out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double s = 0.0;
@@ -1405,5 +1434,38 @@ static double energy_impl(Ctx& c) {
 }
 
 double particles_energy(Ctx& c) { return c.fp32 ? energy_impl<float>(c) : energy_impl<double>(c); }
+
+template <class R>
+static void moments_impl(Ctx& c, int s, double* out) {
+  auto& b = bins<R>(c)[s];
+  for (int q = 0; q < 10; ++q) out[q] = 0.0;
+  if (!b.cnt.p) return;
+  const int nb = c.ntiles;
+  std::vector<int32_t> cnt(nb);
+  DBuf<double> part;
+  part.alloc((size_t)nb * 9);
+  k_moments<R><<<nb, kBlock, 0, c.stream>>>(ptrs(b.reg), b.start.p, b.cnt.p, part.p);
+  std::vector<double> h((size_t)nb * 9);
+  FOPIS_HIP(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipMemcpyAsync(cnt.data(), b.cnt.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  int64_t n = 0;
+  for (int t = 0; t < nb; ++t) n += cnt[t];
+  out[0] = (double)n;
+  if (n == 0) return;
+  double sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = 0; t < nb; ++t)
+    for (int q = 0; q < 9; ++q) sum[q] += h[(size_t)t * 9 + q];
+  for (int q = 0; q < 6; ++q) out[1 + q] = sum[q] / (double)n;
+  for (int q = 0; q < 3; ++q) out[7 + q] = sum[6 + q] / (double)n - out[4 + q] * out[4 + q];
+}
+
+// count, mean x, y, z, mean vx, vy, vz, variance of vx, vy, vz of species s
+void particles_moments(Ctx& c, int s, double* out) {
+  if (c.fp32)
+    moments_impl<float>(c, s, out);
+  else
+    moments_impl<double>(c, s, out);
+}
 
 }  // namespace fopis

@@ -31,6 +31,7 @@ EXPORTS = [
     "fopis_guess_set_mode", "fopis_guess_apply", "fopis_guess_dataset_alloc", "fopis_guess_dataset_store",
     "fopis_guess_dataset_get", "fopis_guess_dataset_set", "fopis_guess_grad", "fopis_guess_train",
     "fopis_guess_eval", "fopis_guess_synth_boxes", "fopis_nn_conv3d", "fopis_nn_conv3d_backward",
+    "fopis_first_solve_cycles", "fopis_moments",
 ]
 
 
@@ -242,6 +243,13 @@ class Context:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.field_shape(which))
         self._ok(self.L.fopis_set_field(self.h, which, s, _dp(a)))
 
+    def moments(self, s: int) -> dict:
+        """count, mean position, mean velocity and velocity variance of species s (FP64)."""
+        out = np.zeros(10)
+        self.L.fopis_moments.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        self._ok(self.L.fopis_moments(self.h, s, _dp(out)))
+        return {"count": out[0], "mean_x": out[1:4], "mean_v": out[4:7], "var_v": out[7:10]}
+
     def energy(self):
         out = np.zeros(2)
         self._ok(self.L.fopis_energy(self.h, _dp(out)))
@@ -307,8 +315,12 @@ def _bind_stats(L):
 
 
 def solver_stats(ctx: "Context") -> dict:
-    """Poisson cycles/relres of the last solve and totals since reset_timers."""
+    """Poisson cycles/relres of the last solve and totals since reset_timers
+    (first_cycles: summed over the first solve of each step)."""
     _bind_stats(ctx.L)
-    lc, lr, tc, ts = C.c_int32(), C.c_double(), C.c_int64(), C.c_int64()
+    lc, lr, tc, ts, fc = C.c_int32(), C.c_double(), C.c_int64(), C.c_int64(), C.c_int64()
     ctx._ok(ctx.L.fopis_solver_stats(ctx.h, C.byref(lc), C.byref(lr), C.byref(tc), C.byref(ts)))
-    return {"last_cycles": lc.value, "last_relres": lr.value, "total_cycles": tc.value, "total_solves": ts.value}
+    ctx.L.fopis_first_solve_cycles.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    ctx._ok(ctx.L.fopis_first_solve_cycles(ctx.h, C.byref(fc)))
+    return {"last_cycles": lc.value, "last_relres": lr.value, "total_cycles": tc.value, "total_solves": ts.value,
+            "first_cycles": fc.value}

@@ -1,9 +1,9 @@
 """Train the learned Poisson initial guess on pairs the solver generates.
 
-Recipe (the 128^3 learned-guess workload): rho is the deposit of a freshly
-drawn plasma of the given configuration (its species, densities and
-temperatures, drawn on the device) plus 1-3 random axis-aligned conductor
-boxes of 8-32 cells per side whose surface nodes carry a seeded charge;
+Recipe (the 128^3 learned-guess workload): rho is the deposit of a plasma
+of the given configuration (its species, densities and temperatures, drawn
+on the device; one draw serves --plasma-every pairs) plus 1-3 random
+axis-aligned conductor boxes of 8-32 cells per side whose surface nodes carry a seeded charge;
 phi is the multigrid solution of that rho to relative residual 1e-8 from
 zero. Pairs [0, ntrain) train the network, pairs after that are held out.
 Training, inference and the pair store all run in libfopis_hip.so.
@@ -62,21 +62,30 @@ def draw_problem(rng, nx: int, rho_rms: float, smin: int, smax: int):
 
 
 class PairMaker:
-    def __init__(self, host: Host, guess: Guess, seed: int, smin: int, smax: int):
+    """Problem i = plasma draw i // every (a fresh device load of the
+    configuration's species, deposited) + its own seeded boxes."""
+
+    def __init__(self, host: Host, guess: Guess, seed: int, smin: int, smax: int, every: int):
         self.h, self.g, self.ctx = host, guess, host.ctx
-        self.seed, self.smin, self.smax = seed, smin, smax
+        self.seed, self.smin, self.smax, self.every = seed, smin, smax, max(1, every)
         self.zero = np.zeros(self.ctx.field_shape(F.PHI))
-        self.h.reload(seed)
-        self.ctx.update_rho()
-        rho = self.ctx.get_field(F.RHO)
+        self.plasma_id, self.plasma = None, None
+        self._plasma(0)
         n = self.ctx.nx
-        self.rho_rms = float(np.sqrt(np.mean(rho[2:n, 2:n, 2:n] ** 2)))
+        self.rho_rms = float(np.sqrt(np.mean(self.plasma[2:n, 2:n, 2:n] ** 2)))
+
+    def _plasma(self, pid: int):
+        if pid != self.plasma_id:
+            self.h.reload(self.seed + 7 * pid + 1)
+            self.ctx.update_rho()
+            self.plasma = self.ctx.get_field(F.RHO)
+            self.plasma_id = pid
 
     def setup(self, i: int):
         """rho and the boxes of problem i (deterministic in i); phi = 0."""
         rng = np.random.default_rng(self.seed * 1_000_003 + i)
-        self.h.reload(self.seed + 7 * i + 1)
-        self.ctx.update_rho()
+        self._plasma(i // self.every)
+        self.ctx.set_field(F.RHO, self.plasma)
         boxes, charges = draw_problem(rng, self.ctx.nx, self.rho_rms, self.smin, self.smax)
         self.g.synth_boxes(boxes, charges, seed=self.seed + i)
         self.ctx.set_field(F.PHI, self.zero)
@@ -100,6 +109,7 @@ def main(argv=None):
     ap.add_argument("--box-min", type=int, default=8)
     ap.add_argument("--box-max", type=int, default=32)
     ap.add_argument("--eval-problems", type=int, default=16, help="held-out problems for the cycle comparison")
+    ap.add_argument("--plasma-every", type=int, default=10, help="pairs that share one plasma draw")
     ap.add_argument("--init", default="", help="start from these weights (.npz)")
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
@@ -115,7 +125,7 @@ def main(argv=None):
         g.set_params(z["params"], float(z["out_scale"]))
     ntot = args.ntrain + args.nheld
     g.dataset_alloc(ntot)
-    pm = PairMaker(h, g, args.seed, args.box_min, args.box_max)
+    pm = PairMaker(h, g, args.seed, args.box_min, args.box_max, args.plasma_every)
     log(f"grid {g.grid}, {g.nparams} parameters, rho rms {pm.rho_rms:.4g}, setup {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     cycles = []

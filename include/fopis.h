@@ -200,6 +200,16 @@ int fopis_reset_timers(fopis_ctx* ctx);
  * RootGridField.cpp:86-88). Any pointer may be NULL. */
 int fopis_solver_stats(fopis_ctx* ctx, int32_t* last_cycles, double* last_relres, int64_t* total_cycles,
                        int64_t* total_solves);
+/* Phase-space moments of species s in FP64 over its live particles:
+ * out[0] count, out[1..3] mean x, y, z, out[4..6] mean vx, vy, vz,
+ * out[7..9] variance of vx, vy, vz (the particle side of the FP32-vs-FP64
+ * comparison; the reference reports energies only, Grid::getParticleEnergy,
+ * GridParticleTreatment.cpp:104-118). */
+int fopis_moments(fopis_ctx* ctx, int s, double* out);
+/* Multigrid cycles summed over the first Poisson solve of each main_loop_es
+ * step since fopis_reset_timers (the solve the learned guess starts; phase
+ * "poisson_first" of fopis_phase_time times guess + that solve). */
+int fopis_first_solve_cycles(fopis_ctx* ctx, int64_t* total_cycles);
 /* Blocks until the context's stream is idle. */
 int fopis_synchronize(fopis_ctx* ctx);
 

@@ -180,3 +180,31 @@ def test_trained_guess_lowers_multigrid_cycles():
     print(f"[guess] multigrid cycles to 1e-8: from zero {zero}, from the network {guessed}")
     assert np.mean(guessed) < np.mean(zero)
     ctx.close()
+
+
+def test_step_with_guess_matches_step_without():
+    """main_loop_es with the guess on (mode 1) starts the step's first solve
+    from the network; multigrid still iterates to 1e-8, so phi agrees with the
+    warm-started step to the solver's tolerance even for an untrained network."""
+    from oracle import refcpu
+    from tests.cases import device_desc, load_config, oracle_params
+
+    gr = refcpu.Grid(oracle_params(load_config("c1"), nx=33, ppc=4))
+    gr.call("rc_load_initial_particles")
+    a, b = F.Context(device_desc(gr)), F.Context(device_desc(gr))
+    for ctx in (a, b):
+        for s in range(gr.S):
+            ctx.set_particles(s, gr.get_particles(s))
+        ctx.initialize()
+    for q in range(3):
+        b.set_field(F.EXREF + q, a.get_field(F.EXREF + q))
+    b.set_field(F.PHI, a.get_field(F.PHI))
+    g = GU.Guess(b, seed=4)
+    g.set_mode(1)
+    a.main_loop_es()
+    b.main_loop_es()
+    assert F.solver_stats(b)["last_relres"] <= 1e-8
+    pa, pb = a.get_field(F.PHI), b.get_field(F.PHI)
+    assert np.max(np.abs(pa - pb)) <= 1e-6 * np.max(np.abs(pa))
+    a.close()
+    b.close()

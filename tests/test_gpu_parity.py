@@ -278,3 +278,19 @@ def test_fp32_fused_step_deposit():
         a, b = ctx.get_field(F.RHO, s + 1), g.get_field(refcpu.RHO, s + 1)
         # each particle's eight FP32 weights carry ~3 roundings of 2^-24
         assert np.max(np.abs(a - b)) <= 2e-6 * np.max(np.abs(b)), s
+
+
+def test_moments_match_host_reduction():
+    g = _oracle(ppc=4)
+    for prec in (F.FP64, F.FP32):
+        ctx = _ctx(g, precision=prec)
+        _upload(ctx, g)
+        for s in range(g.S):
+            d = ctx.get_particles(s)
+            m = ctx.moments(s)
+            a = np.stack([np.asarray(d[k], np.float64) for k in ("x", "y", "z", "vx", "vy", "vz")])
+            assert m["count"] == a.shape[1]
+            np.testing.assert_allclose(m["mean_x"], a[:3].mean(axis=1), rtol=1e-12)
+            np.testing.assert_allclose(m["mean_v"], a[3:].mean(axis=1), rtol=1e-9, atol=1e-15)
+            np.testing.assert_allclose(m["var_v"], a[3:].var(axis=1), rtol=1e-9)
+        ctx.close()
