@@ -200,6 +200,7 @@ struct Ctx {
   int mg_tail = 0;            // first level solved by the single-workgroup LDS kernel
   size_t mg_tail_bytes = 0;   // its LDS bytes
   size_t mg_tail_esz = 0;     // element size the tail was planned for
+  bool mg_tail15 = false;     // the tail is the cubic 15 -> 7 -> 3 -> 1 chain
   size_t lds_max = 65536;     // LDS bytes per workgroup on the device
   GuessNet* guess = nullptr;  // owned; released by guess_free
 

@@ -671,10 +671,11 @@ void launch_conv_t(hipStream_t st, const ConvArgs& a) {
   const int64_t nb = (int64_t)div_up(a.n0, kBI) * div_up(a.n1, kBJ) * div_up(a.n2, kBK);
   const size_t lds = conv_lds_bytes(a.cin);
   static size_t granted = 65536;  // dynamic LDS this instance may use (raised once as needed)
+  const size_t most = conv_lds_bytes(64);
   if (lds > granted) {
     FOPIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv<NT, EPI, UP>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds_bytes(64)));
-    granted = conv_lds_bytes(64);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)most));
+    granted = most;
   }
   k_conv<NT, EPI, UP><<<(unsigned)nb, 256, lds, st>>>(a);
 }

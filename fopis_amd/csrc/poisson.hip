@@ -465,6 +465,112 @@ __global__ __launch_bounds__(kTailThreads) void k_coarse_tail(TailArgs<P> A) {
   }
 }
 
+// The same V-cycle for the cubic chain 15 -> 7 -> 3 -> 1 (every grid of
+// 2^k + 1 nodes per axis ends in it), sizes fixed at compile time: index maps
+// become multiplies and each half-sweep visits only its own colour.
+template <int N>
+struct TailDim {
+  static constexpr int S = N + 2, SX = S * S, SY = S, SIZE = S * S * S, H = (N + 1) / 2;
+};
+
+template <class P, int N>
+__device__ void c_smooth(P* e, const P* r, int sweeps, P w1, P w6) {
+  using D = TailDim<N>;
+  for (int it = 0; it < sweeps; ++it)
+    for (int col = 0; col < 2; ++col) {
+      for (int q = threadIdx.x; q < N * N * D::H; q += kTailThreads) {
+        const int a = q / (N * D::H) + 1, b = (q / D::H) % N + 1, m = q % D::H;
+        const int c = (((col + a + b) & 1) ? 1 : 2) + 2 * m;
+        if (c <= N) {
+          const int g = a * D::SX + b * D::SY + c;
+          const P nb = e[g - D::SX] + e[g + D::SX] + e[g - D::SY] + e[g + D::SY] + e[g - 1] + e[g + 1];
+          e[g] = w1 * e[g] + w6 * (nb + r[g]);
+        }
+      }
+      __syncthreads();
+    }
+}
+
+template <class P, int N>
+__device__ void c_cycle(P* s, P w1, P w6, int pre, int post) {
+  using D = TailDim<N>;
+  P* e = s;
+  P* r = s + D::SIZE;
+  if constexpr (N == 1) {
+    if (threadIdx.x == 0) e[D::SX + D::SY + 1] = r[D::SX + D::SY + 1] / (P)6;
+    __syncthreads();
+  } else {
+    constexpr int NC = (N - 1) / 2;
+    using C = TailDim<NC>;
+    P* t = s + 2 * D::SIZE;
+    P* ec = s + 3 * D::SIZE;
+    P* rc = ec + C::SIZE;
+    c_smooth<P, N>(e, r, pre, w1, w6);
+    for (int q = threadIdx.x; q < N * N * N; q += kTailThreads) {
+      const int a = q / (N * N) + 1, b = (q / N) % N + 1, c = q % N + 1;
+      const int g = a * D::SX + b * D::SY + c;
+      t[g] = r[g] - ((P)6 * e[g] - (e[g - D::SX] + e[g + D::SX] + e[g - D::SY] + e[g + D::SY] + e[g - 1] + e[g + 1]));
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < NC * NC * NC; q += kTailThreads) {
+      const int a = q / (NC * NC) + 1, b = (q / NC) % NC + 1, c = q % NC + 1;
+      const int f = (2 * a) * D::SX + (2 * b) * D::SY + 2 * c;
+      P acc = 0;
+#pragma unroll
+      for (int da = -1; da <= 1; ++da)
+#pragma unroll
+        for (int db = -1; db <= 1; ++db)
+#pragma unroll
+          for (int dc = -1; dc <= 1; ++dc) acc += fw_weight<P>(da, db, dc) * t[f + da * D::SX + db * D::SY + dc];
+      const int gc = a * C::SX + b * C::SY + c;
+      rc[gc] = acc * (P)0.5;
+      ec[gc] = (P)0;
+    }
+    __syncthreads();
+    c_cycle<P, NC>(ec, w1, w6, pre, post);
+    for (int q = threadIdx.x; q < N * N * N; q += kTailThreads) {
+      const int a = q / (N * N) + 1, b = (q / N) % N + 1, c = q % N + 1;
+      const int a0 = a >> 1, a1 = (a + 1) >> 1, b0 = b >> 1, b1 = (b + 1) >> 1, c0 = c >> 1, c1 = (c + 1) >> 1;
+      auto E = [&](int x, int y, int z) { return ec[x * C::SX + y * C::SY + z]; };
+      const P v = (E(a0, b0, c0) + E(a1, b0, c0) + E(a0, b1, c0) + E(a1, b1, c0) + E(a0, b0, c1) + E(a1, b0, c1) +
+                   E(a0, b1, c1) + E(a1, b1, c1)) *
+                  (P)0.125;
+      e[a * D::SX + b * D::SY + c] += v;
+    }
+    __syncthreads();
+    c_smooth<P, N>(e, r, post, w1, w6);
+  }
+}
+
+template <int N>
+constexpr size_t chain_elems() {
+  if constexpr (N == 1)
+    return 2 * TailDim<1>::SIZE;
+  else
+    return 3 * TailDim<N>::SIZE + chain_elems<(N - 1) / 2>();
+}
+
+template <class P>
+__global__ __launch_bounds__(kTailThreads) void k_coarse_tail15(TailArgs<P> A) {
+  if (*A.done) return;
+  extern __shared__ __attribute__((aligned(16))) char tail_lds[];
+  P* s = reinterpret_cast<P*>(tail_lds);
+  using D = TailDim<15>;
+  constexpr int total = (int)chain_elems<15>();
+  for (int q = threadIdx.x; q < total; q += kTailThreads) s[q] = (P)0;
+  __syncthreads();
+  for (int q = threadIdx.x; q < 15 * 15 * 15; q += kTailThreads) {
+    const int a = q / 225 + 1, b = (q / 15) % 15 + 1, c = q % 15 + 1;
+    s[D::SIZE + a * D::SX + b * D::SY + c] = A.gr[a * A.gsx + b * A.gsy + c];
+  }
+  __syncthreads();
+  c_cycle<P, 15>(s, A.w1, A.w6, A.pre, A.post);
+  for (int q = threadIdx.x; q < 15 * 15 * 15; q += kTailThreads) {
+    const int a = q / 225 + 1, b = (q / 15) % 15 + 1, c = q % 15 + 1;
+    A.ge[a * A.gsx + b * A.gsy + c] = s[a * D::SX + b * D::SY + c];
+  }
+}
+
 // Face conditions on phi (Field.cpp:10-96): D -> 0, N -> copy the neighbour.
 __global__ void k_face(double* phi, int cx, int cy, int cz, int axis, int side, int neumann) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -554,6 +660,15 @@ void plan_tail(Ctx& c, size_t esz) {
     c.mg_tail = l;
     c.mg_tail_bytes = bytes;
   }
+  // the compile-time 15 -> 7 -> 3 -> 1 kernel when the tail is exactly that chain
+  c.mg_tail15 = false;
+  if (c.mg_tail < nl && nl - c.mg_tail == 4) {
+    bool ok = true;
+    const int want[4] = {15, 7, 3, 1};
+    for (int q = 0; q < 4; ++q)
+      for (int a = 0; a < 3; ++a) ok = ok && c.mg[c.mg_tail + q].n[a] == want[q];
+    c.mg_tail15 = ok;
+  }
 }
 
 template <class P>
@@ -609,7 +724,10 @@ void vcycle(Ctx& c, int l, const int* done) {
   const double w = c.d.mg_omega;
   if (l == c.mg_tail) {
     TailArgs<P> A = tail_args<P>(c, done);
-    k_coarse_tail<P><<<1, kTailThreads, c.mg_tail_bytes, c.stream>>>(A);
+    if (c.mg_tail15)
+      k_coarse_tail15<P><<<1, kTailThreads, c.mg_tail_bytes, c.stream>>>(A);
+    else
+      k_coarse_tail<P><<<1, kTailThreads, c.mg_tail_bytes, c.stream>>>(A);
     return;
   }
   if (l == (int)c.mg.size() - 1) {
@@ -641,7 +759,7 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   const double* rho = rho0 + off;
   P* r0 = R_<P>(m0);
   const double rc = 1.0 / c.d.eps0;  // pow(dx, 2) / eps0 with dx = 1
-  const int nb = 512;
+  const int nb = 2048;  // workgroups of the level-0 residual passes (8 per CU)
   c.red.alloc(std::max<size_t>(c.red.n, 2 * nb + 4));
   double* part = c.red.p;
   double* dst = c.red.p + 2 * nb;
@@ -650,9 +768,12 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   if (c.mg_tail_esz != sizeof(P)) {
     plan_tail(c, sizeof(P));
     c.mg_tail_esz = sizeof(P);
-    if (c.mg_tail < (int)c.mg.size() && c.mg_tail_bytes > 65536)
+    if (c.mg_tail < (int)c.mg.size() && c.mg_tail_bytes > 65536) {
       FOPIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarse_tail<P>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.mg_tail_bytes));
+      FOPIS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarse_tail15<P>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.mg_tail_bytes));
+    }
   }
 
   // level-0 FP64 residual kept by the cycle updates (the level's own array when P is double)
