@@ -495,6 +495,11 @@ int fopis_guess_set_mode(fopis_ctx* ctx, int mode) {
   return guarded([&] { guess_set_mode(ctx->c, mode); });
 }
 
+int fopis_guess_set_loss(fopis_ctx* ctx, int kind) {
+  NEED_CTX
+  return guarded([&] { guess_set_loss(ctx->c, kind); });
+}
+
 int fopis_guess_apply(fopis_ctx* ctx) {
   NEED_CTX
   return guarded([&] { guess_apply(ctx->c); });

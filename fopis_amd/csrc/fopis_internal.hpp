@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -90,12 +91,61 @@ struct Soa {
     for (auto& a : c) a.release();
   }
   size_t size() const { return c[0].n; }
+  // grow-only: keeps the buffers when they already hold m elements
+  void reserve(size_t m) {
+    if (size() < m) alloc(m);
+  }
 };
 
 // Raw pointers of a SoA set, passed to kernels by value.
 template <class R>
 struct SoaPtr {
   R* c[6];
+};
+
+// Growable pinned host staging of particles added on the host (six FP64
+// components), so that their upload is an asynchronous DMA.
+struct PinSoa {
+  double* c[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t n = 0, cap = 0;
+  PinSoa() = default;
+  PinSoa(const PinSoa&) = delete;
+  PinSoa& operator=(const PinSoa&) = delete;
+  PinSoa(PinSoa&& o) noexcept : n(o.n), cap(o.cap) {
+    for (int q = 0; q < 6; ++q) {
+      c[q] = o.c[q];
+      o.c[q] = nullptr;
+    }
+    o.n = o.cap = 0;
+  }
+  ~PinSoa() { release(); }
+  void release() {
+    for (auto& p : c)
+      if (p) {
+        (void)hipHostFree(p);
+        p = nullptr;
+      }
+    n = cap = 0;
+  }
+  // Appends m particles; `st` is synchronised before an old buffer is freed.
+  void append(const double* const* h, size_t m, hipStream_t st) {
+    if (n + m > cap) {
+      size_t nc = cap ? 2 * cap : ((size_t)1 << 16);
+      while (nc < n + m) nc *= 2;
+      if (st) FOPIS_HIP(hipStreamSynchronize(st));
+      for (auto& p : c) {
+        double* np = nullptr;
+        if (hipHostMalloc(reinterpret_cast<void**>(&np), nc * sizeof(double), hipHostMallocDefault) != hipSuccess)
+          throw Error(FOPIS_E_NOMEM, "pinned host allocation failed");
+        if (n) std::memcpy(np, p, n * sizeof(double));
+        if (p) (void)hipHostFree(p);
+        p = np;
+      }
+      cap = nc;
+    }
+    for (int q = 0; q < 6; ++q) std::memcpy(c[q] + n, h[q], m * sizeof(double));
+    n += m;
+  }
 };
 
 // Device bookkeeping of one species.
@@ -110,10 +160,12 @@ struct SpeciesBins {
   DBuf<int64_t> ostart;  // [ntiles]
   DBuf<int32_t> ocapa;   // [ntiles]
   DBuf<int32_t> ocnt;    // [ntiles]
-  Soa<double> inbox;     // staged new particles (FP64 from the host), sorted by tile
+  Soa<double> inbox;     // staged new particles (FP64 from the host), sorted by tile (grow-only)
+  Soa<double> inbox_raw; // their upload in arrival order (grow-only)
   DBuf<int32_t> ibeg;    // [ntiles+1]
   int64_t inbox_n = 0;
-  std::vector<double> host_inbox[6];  // staged on the host until the next deposit
+  PinSoa host_inbox;     // staged on the host (pinned) until the next deposit
+  bool upload_pending = false;  // an upload from host_inbox may still be queued on the stream
   Soa<R> stray;          // particles that did not fit a bin this step
   int64_t stray_cap = 0;
   int64_t total_cap = 0;
@@ -254,6 +306,7 @@ int64_t guess_num_params(Ctx& c);
 void guess_get_params(Ctx& c, float* out, double* k_out);
 void guess_set_params(Ctx& c, const float* in, double k_out);
 void guess_set_mode(Ctx& c, int mode);
+void guess_set_loss(Ctx& c, int kind);
 int guess_mode(const Ctx& c);
 void guess_apply(Ctx& c);
 void guess_dataset_alloc(Ctx& c, int capacity);

@@ -536,6 +536,59 @@ __global__ __launch_bounds__(256) void k_loss(const float* out, const uint16_t* 
   if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// Residual loss 0.5/N * sum q^2 with q = A (k out) - rho_hat over voxels of
+// interior nodes (A the 7-point operator 6 u - sum of neighbours, the output
+// taken as 0 outside those voxels): the initial relative residual the
+// multigrid starts from. First pass: q and the loss partials.
+__global__ __launch_bounds__(256) void k_res_loss(const float* out, const uint16_t* rho_hat, float k, Geo G,
+                                                  float* q, double* part) {
+  __shared__ double s[256];
+  const int64_t n = (int64_t)G.n0 * G.n1 * G.n2;
+  double acc = 0.0;
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256) {
+    const int kk = (int)(v % G.n2), j = (int)((v / G.n2) % G.n1), i = (int)(v / ((int64_t)G.n1 * G.n2));
+    float qv = 0.f;
+    if (i < G.vn0 && j < G.vn1 && kk < G.vn2) {
+      auto o = [&](int a, int b, int c) -> float {
+        if (a < 0 || b < 0 || c < 0 || a >= G.vn0 || b >= G.vn1 || c >= G.vn2) return 0.f;
+        return out[((int64_t)a * G.n1 + b) * G.n2 + c];
+      };
+      const float au = 6.f * out[v] - (o(i - 1, j, kk) + o(i + 1, j, kk) + o(i, j - 1, kk) + o(i, j + 1, kk) +
+                                       o(i, j, kk - 1) + o(i, j, kk + 1));
+      qv = k * au - bf2f(rho_hat[v]);
+    }
+    q[v] = qv;
+    acc += 0.5 * (double)qv * qv;
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o2 = 128; o2 > 0; o2 >>= 1) {
+    if (threadIdx.x < o2) s[threadIdx.x] += s[threadIdx.x + o2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// Second pass: d loss / d out = (k/N) A q (A is symmetric), into channel 0 of g [voxel][16].
+__global__ void k_res_grad(const float* q, float k_over_n, Geo G, uint16_t* g) {
+  const int64_t n = (int64_t)G.n0 * G.n1 * G.n2;
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int kk = (int)(v % G.n2), j = (int)((v / G.n2) % G.n1), i = (int)(v / ((int64_t)G.n1 * G.n2));
+  float gv = 0.f;
+  if (i < G.vn0 && j < G.vn1 && kk < G.vn2) {
+    auto Q = [&](int a, int b, int c) -> float {
+      if (a < 0 || b < 0 || c < 0 || a >= G.vn0 || b >= G.vn1 || c >= G.vn2) return 0.f;
+      return q[((int64_t)a * G.n1 + b) * G.n2 + c];
+    };
+    gv = k_over_n * (6.f * q[v] - (Q(i - 1, j, kk) + Q(i + 1, j, kk) + Q(i, j - 1, kk) + Q(i, j + 1, kk) +
+                                   Q(i, j, kk - 1) + Q(i, j, kk + 1)));
+  }
+  u32x4 z = {(uint32_t)f2bf(gv), 0u, 0u, 0u};
+  *reinterpret_cast<u32x4*>(g + v * 16) = z;
+  *reinterpret_cast<u32x4*>(g + v * 16 + 8) = (u32x4){0u, 0u, 0u, 0u};
+}
+
 __global__ void k_sum_into(const double* part, int nb, double scale, double* out) {
   if (threadIdx.x != 0) return;
   double t = 0.0;
@@ -628,6 +681,8 @@ struct GuessNet {
   DBuf<uint8_t> ds_mask;
   std::vector<double> ds_phi_rms;
   int mode = 0;
+  int loss_kind = 0;  // 0: potential MSE, 1: initial multigrid residual
+  DBuf<float> q;      // residual of the guess (loss_kind 1)
   uint64_t seed = 1;
 };
 
@@ -924,13 +979,25 @@ void features_from_slot(Ctx& c, GuessNet& g, int slot) {
                                                          g.X0.p, n);
 }
 
-double loss_and_grad(Ctx& c, GuessNet& g, int slot, bool sync) {
+// Loss of pair `slot` on g.OUT into *dst (device) and its gradient into g.gOUT.
+void launch_loss(Ctx& c, GuessNet& g, int slot, double* dst) {
   const int64_t n = nvox_of(g.lv[0]);
   const int nb = 256;
   const double nvalid = (double)g.G.vn0 * g.G.vn1 * g.G.vn2;
-  k_loss<<<nb, 256, 0, c.stream>>>(g.OUT.p, g.ds_phi.p + (size_t)slot * n, (float)(1.0 / g.k_out), g.G,
-                                   (float)(1.0 / nvalid), g.gOUT.p, g.dpart.p);
-  k_sum_into<<<1, 1, 0, c.stream>>>(g.dpart.p, nb, 1.0 / nvalid, g.loss_acc.p);
+  if (g.loss_kind == 1) {
+    if (!g.q.p) g.q.alloc(n);
+    k_res_loss<<<nb, 256, 0, c.stream>>>(g.OUT.p, g.ds_rho.p + (size_t)slot * n, (float)g.k_out, g.G, g.q.p,
+                                         g.dpart.p);
+    k_res_grad<<<div_up(n, 256), 256, 0, c.stream>>>(g.q.p, (float)(g.k_out / nvalid), g.G, g.gOUT.p);
+  } else {
+    k_loss<<<nb, 256, 0, c.stream>>>(g.OUT.p, g.ds_phi.p + (size_t)slot * n, (float)(1.0 / g.k_out), g.G,
+                                     (float)(1.0 / nvalid), g.gOUT.p, g.dpart.p);
+  }
+  k_sum_into<<<1, 1, 0, c.stream>>>(g.dpart.p, nb, 1.0 / nvalid, dst);
+}
+
+double loss_and_grad(Ctx& c, GuessNet& g, int slot, bool sync) {
+  launch_loss(c, g, slot, g.loss_acc.p);
   if (!sync) return 0.0;
   double l = 0.0;
   FOPIS_HIP(hipMemcpyAsync(&l, g.loss_acc.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -1060,6 +1127,11 @@ void guess_set_params(Ctx& c, const float* in, double k_out) {
   g.k_out = k_out;
   pack_all(c, g);
   FOPIS_HIP(hipStreamSynchronize(c.stream));
+}
+
+void guess_set_loss(Ctx& c, int kind) {
+  if (kind != 0 && kind != 1) throw Error(FOPIS_E_ARG, "loss kind must be 0 or 1");
+  net(c).loss_kind = kind;
 }
 
 void guess_set_mode(Ctx& c, int mode) {
@@ -1201,11 +1273,7 @@ void guess_train(Ctx& c, int nsamples, int epochs, double lr, double lr_end, uin
       const int slot = order[i];
       features_from_slot(c, g, slot);
       forward(c, g, EPI_F32);
-      const int64_t n = nvox_of(g.lv[0]);
-      const double nvalid = (double)g.G.vn0 * g.G.vn1 * g.G.vn2;
-      k_loss<<<256, 256, 0, c.stream>>>(g.OUT.p, g.ds_phi.p + (size_t)slot * n, (float)(1.0 / g.k_out), g.G,
-                                        (float)(1.0 / nvalid), g.gOUT.p, g.dpart.p);
-      k_sum_into<<<1, 1, 0, c.stream>>>(g.dpart.p, 256, 1.0 / nvalid, g.loss_acc.p + 1 + pending);
+      launch_loss(c, g, slot, g.loss_acc.p + 1 + pending);
       ++pending;
       backward(c, g);
       ++g.adam_t;

@@ -287,6 +287,38 @@ __device__ __forceinline__ void block_rank2(bool pa, bool pb, int& ra, int& rb, 
   __syncthreads();
 }
 
+// Ranks of four predicates over the block: a0/b0 of every thread first, then
+// a1/b1 (offsets of the second pair start after the first pair's totals are
+// added by the caller).
+__device__ __forceinline__ void block_rank4(bool a0, bool b0, bool a1, bool b1, int* ra, int* rb, int* ta, int* tb,
+                                            int* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const unsigned long long m[4] = {__ballot(a0), __ballot(b0), __ballot(a1), __ballot(b1)};
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sh[4 * k + w] = __popcll(m[k]);
+  __syncthreads();
+  int off[4] = {0, 0, 0, 0}, tot[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < kBlock / 64; ++q)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int cnt = sh[4 * k + q];
+      if (q < w) off[k] += cnt;
+      tot[k] += cnt;
+    }
+  ra[0] = off[0] + __popcll(m[0] & lt);
+  rb[0] = off[1] + __popcll(m[1] & lt);
+  ra[1] = off[2] + __popcll(m[2] & lt);
+  rb[1] = off[3] + __popcll(m[3] & lt);
+  ta[0] = tot[0];
+  tb[0] = tot[1];
+  ta[1] = tot[2];
+  tb[1] = tot[3];
+  __syncthreads();
+}
+
 template <class R>
 __device__ __forceinline__ void put_stray(const TileArgs<R>& a, const R v[6]) {
   const int slot = atomicAdd(&a.status[0], 1);
@@ -298,6 +330,17 @@ __device__ __forceinline__ void put_stray(const TileArgs<R>& a, const R v[6]) {
   }
 }
 
+template <class R>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
 template <class R, int MODE>
 struct TileSmem {
   static constexpr bool kE = (MODE & M_VEL) != 0;
@@ -306,13 +349,13 @@ struct TileSmem {
   // every segment starts on a 16-byte boundary: the FP64 charge window must be
   // 8-byte aligned in LDS whatever the size of the E window before it
   static constexpr size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
-  static constexpr size_t e_bytes = kE ? align16(sizeof(R) * 3 * kEW * kEW * kEW) : 0;
+  // E window interleaved per node as (x, y, z, unused): one 16-byte LDS read per corner (FP32)
+  static constexpr size_t e_bytes = kE ? align16(sizeof(R) * 4 * kEW * kEW * kEW) : 0;
   // charge windows accumulate in FP64 for both particle precisions: FP32 LDS
   // atomics saturate the LDS data FIFO on gfx950 (SQ_WAIT_INST_LDS 50x the
   // FP64 kernel's), FP64 ones do not
   static constexpr size_t r_bytes = kR ? align16(sizeof(double) * kRW * kRW * kRW) : 0;
-  static constexpr size_t d_bytes = kD ? align16(sizeof(int) * kDW * kDW * kDW) : 0;
-  static constexpr size_t bytes = e_bytes + r_bytes + d_bytes + 16;
+  static constexpr size_t bytes = e_bytes + r_bytes + 16;
   static_assert(e_bytes % 16 == 0 && r_bytes % 16 == 0, "LDS segments must stay 16-byte aligned");
 };
 
@@ -324,10 +367,11 @@ template <class R, int MODE>
 __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   using SM = TileSmem<R, MODE>;
   __shared__ __attribute__((aligned(16))) char smem[SM::bytes];
-  __shared__ int sh_rank[8];
+  __shared__ int sh_rank[16];
   R* sE = reinterpret_cast<R*>(smem);
   double* sR = reinterpret_cast<double*>(smem + SM::e_bytes);
-  int* sD = reinterpret_cast<int*>(smem + SM::e_bytes + SM::r_bytes);
+  // the count window is its own LDS array so that its atomics stay LDS atomics (ds_add_u32)
+  __shared__ int sD[SM::kD ? kDW * kDW * kDW : 1];
 
   const int t = blockIdx.x;
   const int tz = t % a.T[2], ty = (t / a.T[2]) % a.T[1], tx = t / (a.T[2] * a.T[1]);
@@ -345,9 +389,10 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         vy = (R)a.ey[g];
         vz = (R)a.ez[g];
       }
-      sE[q] = vx;
-      sE[ne + q] = vy;
-      sE[2 * ne + q] = vz;
+      sE[4 * q] = vx;
+      sE[4 * q + 1] = vy;
+      sE[4 * q + 2] = vz;
+      sE[4 * q + 3] = (R)0;
     }
   }
   if constexpr (SM::kR) {
@@ -372,7 +417,10 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   const int ocap = ((MODE & M_BIN) != 0) ? a.ocapa[t] : 0;
   const int rcap = a.capa[t];
 
-  // the next chunk is loaded while the current one is processed
+  // PPT particles per thread and iteration (FP32: two chunks of kBlock, whose
+  // loads are in flight together; one block scan ranks both), the next
+  // iteration's particles loaded while the current ones are processed
+  constexpr int PPT = sizeof(R) == 4 ? 2 : 1;
   auto fetch = [&](int idx, R* q) {
     if (idx < nreg) {
 #pragma unroll
@@ -385,203 +433,240 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
       for (int c = 0; c < 6; ++c) q[c] = (R)0;
     }
   };
-  R nxt[6];
-  fetch(threadIdx.x, nxt);
-  for (int c0 = 0; c0 < ntot; c0 += kBlock) {
-    const int idx = c0 + threadIdx.x;
+  R nxt[PPT][6];
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) fetch(u * kBlock + threadIdx.x, nxt[u]);
+
+  // one particle through velocity, position, cell, density, capture and
+  // deposit; returns whether it is still valid (its cell in ci, cj, ck)
+  auto work = [&](R (&p)[6], int idx, int& ci, int& cj, int& ck) -> bool {
     const bool act = idx < ntot;
     const bool from_in = idx >= nreg;
-    R p[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) p[c] = nxt[c];
-    fetch(idx + kBlock, nxt);
     bool valid = act && !isnan(p[0]);
 
-    // ---- velocity: trilinear gather of nodal E, Boris rotation with t = b
-    if constexpr ((MODE & M_VEL) != 0) {
-      if (valid && !from_in) {
-        const R x = p[0], y = p[1], z = p[2];
-        int ci = (int)floor(x), cj = (int)floor(y), ck = (int)floor(z);
-        ci = min(ci, a.N[0] - 1);
-        cj = min(cj, a.N[1] - 1);
-        ck = min(ck, a.N[2] - 1);
-        const R dx1 = x - (R)ci, dy1 = y - (R)cj, dz1 = z - (R)ck;
-        const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
-        const R qm = (R)a.qm;
-        const R v1 = qm * dx2 * dy2 * dz2;
-        const R v2 = qm * dx1 * dy2 * dz2;
-        const R v3 = qm * dx2 * dy1 * dz2;
-        const R v4 = qm * dx1 * dy1 * dz2;
-        const R v5 = qm * dx2 * dy2 * dz1;
-        const R v6 = qm * dx1 * dy2 * dz1;
-        const R v7 = qm * dx2 * dy1 * dz1;
-        const R v8 = qm * dx1 * dy1 * dz1;
-        const int li = ci - o0, lj = cj - o1, lk = ck - o2;
-        R e[3];
-        if (li >= 0 && li < kTile && lj >= 0 && lj < kTile && lk >= 0 && lk < kTile) {
-          constexpr int ne = kEW * kEW * kEW;
-          const int b0 = (li * kEW + lj) * kEW + lk;
+      // ---- velocity: trilinear gather of nodal E, Boris rotation with t = b
+      if constexpr ((MODE & M_VEL) != 0) {
+        if (valid && !from_in) {
+          const R x = p[0], y = p[1], z = p[2];
+          int ci = (int)floor(x), cj = (int)floor(y), ck = (int)floor(z);
+          ci = min(ci, a.N[0] - 1);
+          cj = min(cj, a.N[1] - 1);
+          ck = min(ck, a.N[2] - 1);
+          const R dx1 = x - (R)ci, dy1 = y - (R)cj, dz1 = z - (R)ck;
+          const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
+          const R qm = (R)a.qm;
+          const R v1 = qm * dx2 * dy2 * dz2;
+          const R v2 = qm * dx1 * dy2 * dz2;
+          const R v3 = qm * dx2 * dy1 * dz2;
+          const R v4 = qm * dx1 * dy1 * dz2;
+          const R v5 = qm * dx2 * dy2 * dz1;
+          const R v6 = qm * dx1 * dy2 * dz1;
+          const R v7 = qm * dx2 * dy1 * dz1;
+          const R v8 = qm * dx1 * dy1 * dz1;
+          const int li = ci - o0, lj = cj - o1, lk = ck - o2;
+          R e[3];
+          if (li >= 0 && li < kTile && lj >= 0 && lj < kTile && lk >= 0 && lk < kTile) {
+            const int b0 = (li * kEW + lj) * kEW + lk;
+            // the eight corner nodes, each (x, y, z) in one vector read; same sums as the reference order
+            using V4 = typename Vec4<R>::type;
+            const V4* E4 = reinterpret_cast<const V4*>(sE) + b0;
+            const V4 n1 = E4[0], n2 = E4[kEW * kEW], n3 = E4[kEW], n4 = E4[kEW * kEW + kEW];
+            const V4 n5 = E4[1], n6 = E4[kEW * kEW + 1], n7 = E4[kEW + 1], n8 = E4[kEW * kEW + kEW + 1];
+            e[0] = v1 * n1.x + v2 * n2.x + v3 * n3.x + v4 * n4.x + v5 * n5.x + v6 * n6.x + v7 * n7.x + v8 * n8.x;
+            e[1] = v1 * n1.y + v2 * n2.y + v3 * n3.y + v4 * n4.y + v5 * n5.y + v6 * n6.y + v7 * n7.y + v8 * n8.y;
+            e[2] = v1 * n1.z + v2 * n2.z + v3 * n3.z + v4 * n4.z + v5 * n5.z + v6 * n6.z + v7 * n7.z + v8 * n8.z;
+          } else {  // outside the tile (a bin left stale by an interrupted step): global gather
+            const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
+            const size_t g = (size_t)(ci + 1) * sI + (size_t)(cj + 1) * sJ + (ck + 1);
+            const double* Es[3] = {a.ex, a.ey, a.ez};
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const R* E = sE + c * ne + b0;
-            e[c] = v1 * E[0] + v2 * E[kEW * kEW] + v3 * E[kEW] + v4 * E[kEW * kEW + kEW] + v5 * E[1] +
-                   v6 * E[kEW * kEW + 1] + v7 * E[kEW + 1] + v8 * E[kEW * kEW + kEW + 1];
+            for (int c = 0; c < 3; ++c) {
+              const double* E = Es[c] + g;
+              e[c] = v1 * (R)E[0] + v2 * (R)E[sI] + v3 * (R)E[sJ] + v4 * (R)E[sI + sJ] + v5 * (R)E[1] +
+                     v6 * (R)E[sI + 1] + v7 * (R)E[sJ + 1] + v8 * (R)E[sI + sJ + 1];
+            }
           }
-        } else {  // outside the tile (a bin left stale by an interrupted step): global gather
-          const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
-          const size_t g = (size_t)(ci + 1) * sI + (size_t)(cj + 1) * sJ + (ck + 1);
-          const double* Es[3] = {a.ex, a.ey, a.ez};
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const double* E = Es[c] + g;
-            e[c] = v1 * (R)E[0] + v2 * (R)E[sI] + v3 * (R)E[sJ] + v4 * (R)E[sI + sJ] + v5 * (R)E[1] +
-                   v6 * (R)E[sI + 1] + v7 * (R)E[sJ + 1] + v8 * (R)E[sI + sJ + 1];
-          }
+          const R bx = (R)a.bx, by = (R)a.by, bz = (R)a.bz, boris = (R)a.boris;
+          const R vx1 = p[3] + e[0], vy1 = p[4] + e[1], vz1 = p[5] + e[2];
+          const R vxt = vx1 + vy1 * bz - vz1 * by;
+          const R vyt = vy1 + vz1 * bx - vx1 * bz;
+          const R vzt = vz1 + vx1 * by - vy1 * bx;
+          p[3] = vx1 + (e[0] + boris * (vyt * bz - vzt * by)) * (R)1;
+          p[4] = vy1 + (e[1] + boris * (vzt * bx - vxt * bz)) * (R)1;
+          p[5] = vz1 + (e[2] + boris * (vxt * by - vyt * bx)) * (R)1;
         }
-        const R bx = (R)a.bx, by = (R)a.by, bz = (R)a.bz, boris = (R)a.boris;
-        const R vx1 = p[3] + e[0], vy1 = p[4] + e[1], vz1 = p[5] + e[2];
-        const R vxt = vx1 + vy1 * bz - vz1 * by;
-        const R vyt = vy1 + vz1 * bx - vx1 * bz;
-        const R vzt = vz1 + vx1 * by - vy1 * bx;
-        p[3] = vx1 + (e[0] + boris * (vyt * bz - vzt * by)) * (R)1;
-        p[4] = vy1 + (e[1] + boris * (vzt * bx - vxt * bz)) * (R)1;
-        p[5] = vz1 + (e[2] + boris * (vxt * by - vyt * bx)) * (R)1;
       }
-    }
 
-    // ---- position: x += v; leaving a physical face invalidates
-    if constexpr ((MODE & M_POS) != 0) {
-      if (valid && !from_in) {
-        p[0] = p[0] + p[3];
-        p[1] = p[1] + p[4];
-        p[2] = p[2] + p[5];
-        if (p[0] < (R)0 || p[0] > (R)a.xmax[0] || p[1] < (R)0 || p[1] > (R)a.xmax[1] || p[2] < (R)0 ||
-            p[2] > (R)a.xmax[2])
+      // ---- position: x += v; leaving a physical face invalidates
+      if constexpr ((MODE & M_POS) != 0) {
+        if (valid && !from_in) {
+          p[0] = p[0] + p[3];
+          p[1] = p[1] + p[4];
+          p[2] = p[2] + p[5];
+          if (p[0] < (R)0 || p[0] > (R)a.xmax[0] || p[1] < (R)0 || p[1] > (R)a.xmax[1] || p[2] < (R)0 ||
+              p[2] > (R)a.xmax[2])
+            valid = false;
+        }
+      }
+
+      ci = 0;
+      cj = 0;
+      ck = 0;
+      if (valid) {
+        ci = (int)floor(p[0]);
+        cj = (int)floor(p[1]);
+        ck = (int)floor(p[2]);
+        if (from_in && (p[0] < (R)0 || p[0] > (R)a.xmax[0] || p[1] < (R)0 || p[1] > (R)a.xmax[1] ||
+                        p[2] < (R)0 || p[2] > (R)a.xmax[2]))
           valid = false;
       }
-    }
 
-    int ci = 0, cj = 0, ck = 0;
-    if (valid) {
-      ci = (int)floor(p[0]);
-      cj = (int)floor(p[1]);
-      ck = (int)floor(p[2]);
-      if (from_in && (p[0] < (R)0 || p[0] > (R)a.xmax[0] || p[1] < (R)0 || p[1] > (R)a.xmax[1] ||
-                      p[2] < (R)0 || p[2] > (R)a.xmax[2]))
-        valid = false;
-    }
-
-    // ---- density: count per reference cell (RootGridField.cpp:1129-1150)
-    if constexpr (SM::kD) {
-      if (valid) {
-        const int wi = ci - o0 + 1, wj = cj - o1 + 1, wk = ck - o2 + 1;
-        if (wi >= 0 && wi < kDW && wj >= 0 && wj < kDW && wk >= 0 && wk < kDW) {
-          atomicAdd(&sD[(wi * kDW + wj) * kDW + wk], 1);
-        } else {
-          const size_t g = ((size_t)(ci + 1) * (a.cy - 1) + (cj + 1)) * (a.cz - 1) + (ck + 1);
-          atomicAdd(&a.dens_g[g], a.size);
-          atomicAdd(&a.status[2], 1);
-        }
-      }
-    }
-
-    // ---- conductor capture, then CIC deposit
-    if constexpr ((MODE & M_DEP) != 0) {
-      if (valid && a.nobj > 0) {
-        const int mi = ci + 1, mj = cj + 1, mk = ck + 1;
-        const uint8_t m = a.mask[((size_t)mi * (a.cy - 1) + mj) * (a.cz - 1) + mk];
-        if (m != 0) {
-          // captured: recorded for the capture kernel (surface charge on the device)
-          const int slot = atomicAdd(&a.status[3], 1);
-          if (slot < a.cap_cap) {
-            double* r = a.cap_rec + (size_t)8 * slot;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) r[c] = (double)p[c];
-            r[6] = (double)(m - 1);
+      // ---- density: count per reference cell (RootGridField.cpp:1129-1150)
+      if constexpr (SM::kD) {
+        if (valid) {
+          const int wi = ci - o0 + 1, wj = cj - o1 + 1, wk = ck - o2 + 1;
+          if (wi >= 0 && wi < kDW && wj >= 0 && wj < kDW && wk >= 0 && wk < kDW) {
+            atomicAdd(&sD[(wi * kDW + wj) * kDW + wk], 1);
           } else {
-            atomicAdd(&a.status[1], 1);
+            const size_t g = ((size_t)(ci + 1) * (a.cy - 1) + (cj + 1)) * (a.cz - 1) + (ck + 1);
+            atomicAdd(&a.dens_g[g], a.size);
+            atomicAdd(&a.status[2], 1);
           }
-          valid = false;
         }
       }
-      if (valid) {
-        const int di = min(ci, a.N[0] - 1), dj = min(cj, a.N[1] - 1), dk = min(ck, a.N[2] - 1);
-        const R dx1 = p[0] - (R)di, dy1 = p[1] - (R)dj, dz1 = p[2] - (R)dk;
-        const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
-        const R q = (R)a.q;
-        const R w[8] = {dx2 * dy2 * dz2 * q, dx1 * dy2 * dz2 * q, dx2 * dy1 * dz2 * q, dx1 * dy1 * dz2 * q,
-                        dx2 * dy2 * dz1 * q, dx1 * dy2 * dz1 * q, dx2 * dy1 * dz1 * q, dx1 * dy1 * dz1 * q};
-        const int wi = di - o0 + 1, wj = dj - o1 + 1, wk = dk - o2 + 1;
-        if (wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW) {
-          double* W = sR + (wi * kRW + wj) * kRW + wk;
-          atomicAdd(W, (double)w[0]);
-          atomicAdd(W + kRW * kRW, (double)w[1]);
-          atomicAdd(W + kRW, (double)w[2]);
-          atomicAdd(W + kRW * kRW + kRW, (double)w[3]);
-          atomicAdd(W + 1, (double)w[4]);
-          atomicAdd(W + kRW * kRW + 1, (double)w[5]);
-          atomicAdd(W + kRW + 1, (double)w[6]);
-          atomicAdd(W + kRW * kRW + kRW + 1, (double)w[7]);
-        } else {
-          const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
-          double* G = a.rho_g + (size_t)(di + 1) * sI + (size_t)(dj + 1) * sJ + (dk + 1);
-          atomicAdd(G, (double)w[0]);
-          atomicAdd(G + sI, (double)w[1]);
-          atomicAdd(G + sJ, (double)w[2]);
-          atomicAdd(G + sI + sJ, (double)w[3]);
-          atomicAdd(G + 1, (double)w[4]);
-          atomicAdd(G + sI + 1, (double)w[5]);
-          atomicAdd(G + sJ + 1, (double)w[6]);
-          atomicAdd(G + sI + sJ + 1, (double)w[7]);
-          atomicAdd(&a.status[2], 1);
+
+      // ---- conductor capture, then CIC deposit
+      if constexpr ((MODE & M_DEP) != 0) {
+        if (valid && a.nobj > 0) {
+          const int mi = ci + 1, mj = cj + 1, mk = ck + 1;
+          const uint8_t m = a.mask[((size_t)mi * (a.cy - 1) + mj) * (a.cz - 1) + mk];
+          if (m != 0) {
+            // captured: recorded for the capture kernel (surface charge on the device)
+            const int slot = atomicAdd(&a.status[3], 1);
+            if (slot < a.cap_cap) {
+              double* r = a.cap_rec + (size_t)8 * slot;
+#pragma unroll
+              for (int c = 0; c < 6; ++c) r[c] = (double)p[c];
+              r[6] = (double)(m - 1);
+            } else {
+              atomicAdd(&a.status[1], 1);
+            }
+            valid = false;
+          }
+        }
+        if (valid) {
+          const int di = min(ci, a.N[0] - 1), dj = min(cj, a.N[1] - 1), dk = min(ck, a.N[2] - 1);
+          const R dx1 = p[0] - (R)di, dy1 = p[1] - (R)dj, dz1 = p[2] - (R)dk;
+          const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
+          const R q = (R)a.q;
+          const R w[8] = {dx2 * dy2 * dz2 * q, dx1 * dy2 * dz2 * q, dx2 * dy1 * dz2 * q, dx1 * dy1 * dz2 * q,
+                          dx2 * dy2 * dz1 * q, dx1 * dy2 * dz1 * q, dx2 * dy1 * dz1 * q, dx1 * dy1 * dz1 * q};
+          const int wi = di - o0 + 1, wj = dj - o1 + 1, wk = dk - o2 + 1;
+          if (wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW) {
+            double* W = sR + (wi * kRW + wj) * kRW + wk;
+            atomicAdd(W, (double)w[0]);
+            atomicAdd(W + kRW * kRW, (double)w[1]);
+            atomicAdd(W + kRW, (double)w[2]);
+            atomicAdd(W + kRW * kRW + kRW, (double)w[3]);
+            atomicAdd(W + 1, (double)w[4]);
+            atomicAdd(W + kRW * kRW + 1, (double)w[5]);
+            atomicAdd(W + kRW + 1, (double)w[6]);
+            atomicAdd(W + kRW * kRW + kRW + 1, (double)w[7]);
+          } else {
+            const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
+            double* G = a.rho_g + (size_t)(di + 1) * sI + (size_t)(dj + 1) * sJ + (dk + 1);
+            atomicAdd(G, (double)w[0]);
+            atomicAdd(G + sI, (double)w[1]);
+            atomicAdd(G + sJ, (double)w[2]);
+            atomicAdd(G + sI + sJ, (double)w[3]);
+            atomicAdd(G + 1, (double)w[4]);
+            atomicAdd(G + sI + 1, (double)w[5]);
+            atomicAdd(G + sJ + 1, (double)w[6]);
+            atomicAdd(G + sI + sJ + 1, (double)w[7]);
+            atomicAdd(&a.status[2], 1);
+          }
         }
       }
-    }
+
+    return valid;
+  };
+
+  for (int c0 = 0; c0 < ntot; c0 += PPT * kBlock) {
+    R p[PPT][6];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) p[u][c] = nxt[u][c];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) fetch(c0 + (PPT + u) * kBlock + threadIdx.x, nxt[u]);
+    bool valid[PPT];
+    int ci[PPT], cj[PPT], ck[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) valid[u] = work(p[u], c0 + u * kBlock + threadIdx.x, ci[u], cj[u], ck[u]);
 
     if constexpr ((MODE & M_BIN) != 0) {
-      int dir = 13;
-      bool far = false;
-      if (valid) {
-        const int bi = min(ci, a.N[0] - 1) / kTile, bj = min(cj, a.N[1] - 1) / kTile,
-                  bk = min(ck, a.N[2] - 1) / kTile;
-        const int d0 = bi - tx, d1 = bj - ty, d2 = bk - tz;
-        if (d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1)
-          far = true;
-        else
-          dir = (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1);
-      }
-      const bool stay = valid && !far && dir == 13;
-      const bool move = valid && !far && dir != 13;
-      int rs, rm, ts, tm;
-      block_rank2(stay, move, rs, rm, ts, tm, sh_rank);  // also orders all reads before the writes
-      if (stay) {
-        const int pos = wptr + rs;
-        if (pos < rcap) {
+      int dir[PPT];
+      bool far[PPT], stay[PPT], move[PPT];
 #pragma unroll
-          for (int c = 0; c < 6; ++c) a.reg.c[c][base + pos] = p[c];
-        } else {
-          put_stray(a, p);
+      for (int u = 0; u < PPT; ++u) {
+        dir[u] = 13;
+        far[u] = false;
+        if (valid[u]) {
+          const int bi = min(ci[u], a.N[0] - 1) / kTile, bj = min(cj[u], a.N[1] - 1) / kTile,
+                    bk = min(ck[u], a.N[2] - 1) / kTile;
+          const int d0 = bi - tx, d1 = bj - ty, d2 = bk - tz;
+          if (d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1)
+            far[u] = true;
+          else
+            dir[u] = (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1);
         }
-      } else if (move) {
-        const int pos = optr + rm;
-        if (pos < ocap) {
-          const int64_t ob = a.ostart[t] + pos;
-#pragma unroll
-          for (int c = 0; c < 6; ++c) a.out.c[c][ob] = p[c];
-          a.odir[ob] = (uint8_t)dir;
-        } else {
-          put_stray(a, p);
-        }
-      } else if (far) {
-        put_stray(a, p);
+        stay[u] = valid[u] && !far[u] && dir[u] == 13;
+        move[u] = valid[u] && !far[u] && dir[u] != 13;
       }
-      wptr = min(wptr + ts, rcap);
-      optr = min(optr + tm, ocap);
+      // ranks in (u, thread) order; the scan's barriers also order all reads before the writes
+      int rs[PPT], rm[PPT], ts[PPT], tm[PPT];
+      if constexpr (PPT == 1)
+        block_rank2(stay[0], move[0], rs[0], rm[0], ts[0], tm[0], sh_rank);
+      else
+        block_rank4(stay[0], move[0], stay[1], move[1], rs, rm, ts, tm, sh_rank);
+      int wb = wptr, ob0 = optr;
+#pragma unroll
+      for (int u = 0; u < PPT; ++u) {
+        if (stay[u]) {
+          const int pos = wb + rs[u];
+          if (pos < rcap) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a.reg.c[c][base + pos] = p[u][c];
+          } else {
+            put_stray(a, p[u]);
+          }
+        } else if (move[u]) {
+          const int pos = ob0 + rm[u];
+          if (pos < ocap) {
+            const int64_t ob = a.ostart[t] + pos;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a.out.c[c][ob] = p[u][c];
+            a.odir[ob] = (uint8_t)dir[u];
+          } else {
+            put_stray(a, p[u]);
+          }
+        } else if (far[u]) {
+          put_stray(a, p[u]);
+        }
+        wb += ts[u];
+        ob0 += tm[u];
+      }
+      wptr = min(wb, rcap);
+      optr = min(ob0, ocap);
     } else if constexpr ((MODE & (M_VEL | M_POS)) != 0) {
-      if (act && !from_in) {
-        if (!valid) p[0] = (R)NAN;  // invalid until the next re-bin drops it
 #pragma unroll
-        for (int c = 0; c < 6; ++c) a.reg.c[c][base + idx] = p[c];
+      for (int u = 0; u < PPT; ++u) {
+        const int idx = c0 + u * kBlock + threadIdx.x;
+        if (idx < ntot && idx < nreg) {
+          if (!valid[u]) p[u][0] = (R)NAN;  // invalid until the next re-bin drops it
+#pragma unroll
+          for (int c = 0; c < 6; ++c) a.reg.c[c][base + idx] = p[u][c];
+        }
       }
     }
   }
@@ -679,6 +764,11 @@ __global__ void k_tile_starts(const uint32_t* keys, int64_t n, int shift, uint32
   for (uint32_t u = tp; u <= t; ++u) first[u] = i;
   if (i == n - 1)
     for (uint32_t u = t + 1; u <= ntiles; ++u) first[u] = n;
+}
+
+__global__ void k_i32_from_i64(const int64_t* in, int32_t* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int32_t)in[i];
 }
 
 __global__ void k_counts_from_starts(const int64_t* first, uint32_t ntiles, uint32_t* cnt) {
@@ -1091,16 +1181,17 @@ void rebin(Ctx& c, int s) {
 template <class R>
 void prepare_inbox(Ctx& c, int s, bool consume) {
   auto& b = bins<R>(c)[s];
-  const int64_t n = (int64_t)b.host_inbox[0].size();
+  const int64_t n = (int64_t)b.host_inbox.n;
   b.inbox_n = n;
   if (n == 0) return;
   hipStream_t st = c.stream;
   Scratch& sc = scratch(c);
   const int nt = c.ntiles;
-  Soa<double> raw;
-  raw.alloc(n);
+  Soa<double>& raw = b.inbox_raw;
+  raw.reserve(n);
   for (int q = 0; q < 6; ++q)
-    FOPIS_HIP(hipMemcpyAsync(raw.c[q].p, b.host_inbox[q].data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+    FOPIS_HIP(hipMemcpyAsync(raw.c[q].p, b.host_inbox.c[q], n * sizeof(double), hipMemcpyHostToDevice, st));
+  b.upload_pending = true;
   TileArgs<double> a{};
   for (int q = 0; q < 3; ++q) {
     a.N[q] = c.Nc[q];
@@ -1120,22 +1211,15 @@ void prepare_inbox(Ctx& c, int s, bool consume) {
   count_sorted_tiles(st, sc.keys.p, n, 0, nt, sc.first.p, sc.cnt.p);
   exclusive_scan_u32(st, sc.cnt.p, nt + 1, sc.first.p, sc.tmp);
   b.ibeg.alloc(nt + 1);
-  b.inbox.alloc(n);
+  b.inbox.reserve(n);
   // inbox in tile order (invalid ones sort past ibeg[nt] and are not copied)
   k_fill_regions<double><<<div_up(n, 256), 256, 0, st>>>(ptrs(b.inbox), sc.first.p, ptrs(raw), sc.keys.p,
                                                          sc.perm.p, n, 0, sc.first.p, (uint32_t)nt);
-  std::vector<int64_t> first(nt + 1);
-  FOPIS_HIP(hipMemcpyAsync(first.data(), sc.first.p, (nt + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  FOPIS_HIP(hipStreamSynchronize(st));
-  std::vector<int32_t> ib(nt + 1);
-  for (int q = 0; q <= nt; ++q) ib[q] = (int32_t)first[q];
-  FOPIS_HIP(hipMemcpyAsync(b.ibeg.p, ib.data(), (nt + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  FOPIS_HIP(hipStreamSynchronize(st));
-  if (consume)
-    for (auto& v : b.host_inbox) {
-      v.clear();
-      v.shrink_to_fit();
-    }
+  k_i32_from_i64<<<div_up(nt + 1, 256), 256, 0, st>>>(sc.first.p, b.ibeg.p, nt + 1);
+  FOPIS_HIP(hipGetLastError());
+  // the pinned staging may be refilled once the upload has been consumed by the
+  // stream (the step synchronises on its status read before the next stage)
+  if (consume) b.host_inbox.n = 0;
 }
 
 template <class R, int MODE>
@@ -1150,6 +1234,7 @@ void after_bin(Ctx& c) {
   std::vector<int32_t> stat(4 * c.S);
   FOPIS_HIP(hipMemcpyAsync(stat.data(), c.status.p, 4 * c.S * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
   FOPIS_HIP(hipStreamSynchronize(c.stream));
+  for (auto& b : bins<R>(c)) b.upload_pending = false;
   for (int s = 0; s < c.S; ++s)
     if (stat[4 * s + 1] > 0) throw Error(FOPIS_E_NOMEM, "stray or capture buffer overflow: particles were lost");
   // surface charge of the captured particles (Spacecraft::distributeInnerParticleCharge)
@@ -1291,7 +1376,7 @@ static int64_t count_impl(Ctx& c, int s) {
   FOPIS_HIP(hipStreamSynchronize(c.stream));
   int64_t n = 0;
   for (int v : cnt) n += v;
-  return n + (int64_t)b.host_inbox[0].size();
+  return n + (int64_t)b.host_inbox.n;
 }
 
 int64_t particles_count(Ctx& c, int s) { return c.fp32 ? count_impl<float>(c, s) : count_impl<double>(c, s); }
@@ -1319,8 +1404,14 @@ void particles_get(Ctx& c, int s, int64_t cap, double* const* h, int64_t* n) {
 }
 
 void particles_stage(Ctx& c, int s, int64_t n, const double* const* h) {
-  auto& hb = c.fp32 ? c.sp32[s].host_inbox : c.sp64[s].host_inbox;
-  for (int q = 0; q < 6; ++q) hb[q].insert(hb[q].end(), h[q], h[q] + n);
+  PinSoa& hb = c.fp32 ? c.sp32[s].host_inbox : c.sp64[s].host_inbox;
+  bool& pending = c.fp32 ? c.sp32[s].upload_pending : c.sp64[s].upload_pending;
+  // an upload from the staging may still be queued: wait for it before refilling
+  if (pending) {
+    FOPIS_HIP(hipStreamSynchronize(c.stream));
+    pending = false;
+  }
+  hb.append(h, (size_t)n, c.stream);
 }
 
 void particles_velocity(Ctx& c) {

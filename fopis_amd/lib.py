@@ -31,7 +31,7 @@ EXPORTS = [
     "fopis_guess_set_mode", "fopis_guess_apply", "fopis_guess_dataset_alloc", "fopis_guess_dataset_store",
     "fopis_guess_dataset_get", "fopis_guess_dataset_set", "fopis_guess_grad", "fopis_guess_train",
     "fopis_guess_eval", "fopis_guess_synth_boxes", "fopis_nn_conv3d", "fopis_nn_conv3d_backward",
-    "fopis_first_solve_cycles", "fopis_moments",
+    "fopis_first_solve_cycles", "fopis_moments", "fopis_guess_set_loss",
 ]
 
 

@@ -110,6 +110,8 @@ def main(argv=None):
     ap.add_argument("--box-max", type=int, default=32)
     ap.add_argument("--eval-problems", type=int, default=16, help="held-out problems for the cycle comparison")
     ap.add_argument("--plasma-every", type=int, default=10, help="pairs that share one plasma draw")
+    ap.add_argument("--loss", type=int, default=1,
+                    help="0: potential MSE; 1: the multigrid's starting residual (sets its cycle count)")
     ap.add_argument("--init", default="", help="start from these weights (.npz)")
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
@@ -120,6 +122,7 @@ def main(argv=None):
     os.unlink(cfg)
     ctx = h.ctx
     g = Guess(ctx, seed=args.seed)
+    g.set_loss(args.loss)
     if args.init:
         z = np.load(args.init, allow_pickle=False)
         g.set_params(z["params"], float(z["out_scale"]))
@@ -187,7 +190,7 @@ def main(argv=None):
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
         np.savez(args.out, params=params, out_scale=np.float64(k), grid=np.array(g.grid, np.int32),
                  recipe=np.array(json.dumps({"config": os.path.basename(args.config), "ntrain": args.ntrain,
-                                             "epochs": args.epochs, "seed": args.seed,
+                                             "epochs": args.epochs, "seed": args.seed, "loss": args.loss,
                                              "boxes": [args.box_min, args.box_max]})))
         res["weights"] = args.out + ".npz"
     h.close()

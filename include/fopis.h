@@ -237,6 +237,10 @@ int fopis_guess_set_params(fopis_ctx* ctx, const float* params, double out_scale
 /* 0: every solve starts from the previous phi (reference behaviour);
  * 1: the first solve of each step starts from the network's guess. */
 int fopis_guess_set_mode(fopis_ctx* ctx, int mode);
+/* Training objective: 0 = 0.5*mean((out - phi_hat/k)^2) (default);
+ * 1 = 0.5*mean((A(k*out) - rho_hat)^2), A the 7-point operator: the relative
+ * residual the multigrid starts from, which is what sets its cycle count. */
+int fopis_guess_set_loss(fopis_ctx* ctx, int kind);
 /* phi at the interior nodes <- network(rho[0], mask). */
 int fopis_guess_apply(fopis_ctx* ctx);
 /* Device dataset of training pairs: rho_hat = rho[0]/rms(rho[0]) (BF16), node
@@ -247,7 +251,7 @@ int fopis_guess_dataset_store(fopis_ctx* ctx, int32_t slot, double* phi_hat_rms)
 int fopis_guess_dataset_get(fopis_ctx* ctx, int32_t slot, uint16_t* rho_hat, uint8_t* mask, uint16_t* phi_hat);
 int fopis_guess_dataset_set(fopis_ctx* ctx, int32_t slot, const uint16_t* rho_hat, const uint8_t* mask,
                             const uint16_t* phi_hat);
-/* Loss 0.5*mean((out - phi_hat/k)^2) over interior voxels and its parameter
+/* Loss (fopis_guess_set_loss) over interior voxels and its parameter
  * gradient on pair `slot` (grad may be NULL); no update. */
 int fopis_guess_grad(fopis_ctx* ctx, int32_t slot, double* loss, float* grad);
 /* Adam over pairs [0, npairs), `epochs` shuffled passes, learning rate on a

@@ -11,7 +11,8 @@ network's arithmetic so that the device kernels can be checked:
   y = sum_t x_shift_t @ W[t] + b;
 - up: nearest-neighbour upsampling by 2 (input read at index >> 1);
 - pool: mean over 2x2x2 blocks;
-- loss: 0.5 * mean over voxels of interior nodes of (out - target/k)^2.
+- loss: 0.5 * mean over voxels of interior nodes of (out - target/k)^2, or of
+  the residual (A(k out) - rho_hat)^2 with A the 7-point operator.
 
 Pins: tests/test_guess_oracle.py checks conv3d against torch.nn.functional
 .conv3d on the CPU in float64, and the hand-written backward of the whole
@@ -141,18 +142,34 @@ def forward(x0, params, round_act=False):
     return out, A
 
 
-def loss_grad(x0, params, target, k, valid, round_act=False):
-    """Loss and flat gradient; valid = (vn0, vn1, vn2) voxels of interior nodes."""
+def laplace7(u, m):
+    """6 u - sum of the 6 neighbours on the voxels m, with u taken as 0 outside m."""
+    u = np.where(m, u, 0.0)
+    p = np.pad(u, 1)
+    nb = (p[:-2, 1:-1, 1:-1] + p[2:, 1:-1, 1:-1] + p[1:-1, :-2, 1:-1] + p[1:-1, 2:, 1:-1] +
+          p[1:-1, 1:-1, :-2] + p[1:-1, 1:-1, 2:])
+    return np.where(m, 6.0 * u - nb, 0.0)
+
+
+def loss_grad(x0, params, target, k, valid, round_act=False, kind=0):
+    """Loss and flat gradient; valid = (vn0, vn1, vn2) voxels of interior nodes.
+    kind 0: 0.5 mean((out - target/k)^2); kind 1: 0.5 mean((A(k out) - rho_hat)^2)
+    with rho_hat = x0[..., 0] and A the 7-point operator (laplace7)."""
     out, A = forward(x0, params, round_act)
     P = unpack(params)
     m = np.zeros(out.shape, bool)
     m[: valid[0], : valid[1], : valid[2]] = True
     n = m.sum()
-    d = np.where(m, out - target / k, 0.0)
-    loss = 0.5 * (d * d).sum() / n
     G = [None] * len(LAYERS)
     gout = np.zeros(out.shape + (16,))
-    gout[..., 0] = d / n
+    if kind == 1:
+        q = np.where(m, laplace7(k * out, m) - x0[..., 0], 0.0)
+        loss = 0.5 * (q * q).sum() / n
+        gout[..., 0] = k / n * laplace7(q, m)
+    else:
+        d = np.where(m, out - target / k, 0.0)
+        loss = 0.5 * (d * d).sum() / n
+        gout[..., 0] = d / n
 
     def back(li, x, gy, y=None):
         ci, co, lvl, up, relu = LAYERS[li]

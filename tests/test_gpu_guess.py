@@ -90,10 +90,12 @@ def _smooth(rng, shape, k=3):
     return f
 
 
-def test_network_gradient_matches_oracle():
+@pytest.mark.parametrize("kind", [0, 1])
+def test_network_gradient_matches_oracle(kind):
     n = 17  # network grid 16^3: levels 16, 8, 4, 2
     ctx = F.Context(_desc(n))
     g = GU.Guess(ctx, seed=5)
+    g.set_loss(kind)
     assert g.grid == (16, 16, 16)
     g.dataset_alloc(1)
     rng = np.random.default_rng(13)
@@ -112,7 +114,8 @@ def test_network_gradient_matches_oracle():
     x0 = np.zeros(g.grid + (8,))
     x0[..., 0] = GU.from_bf16(rho)
     x0[..., 1] = mask
-    rl, rg = unet.loss_grad(x0, P, GU.from_bf16(phi).astype(np.float64), 1.0, (n - 2,) * 3, round_act=True)
+    rl, rg = unet.loss_grad(x0, P, GU.from_bf16(phi).astype(np.float64), 1.0, (n - 2,) * 3, round_act=True,
+                            kind=kind)
     assert abs(loss - rl) <= 2e-2 * abs(rl), (loss, rl)
     for name, ws, _, bs in GU.param_slices()[0]:
         for sl in (ws, bs):
@@ -153,6 +156,7 @@ def test_trained_guess_lowers_multigrid_cycles():
     n = 33  # network grid 32^3
     ctx = F.Context(_desc(n))
     g = GU.Guess(ctx, seed=9)
+    g.set_loss(1)  # train on the multigrid's starting residual
     ntrain, nheld = 48, 8
     g.dataset_alloc(ntrain + nheld)
     rng = np.random.default_rng(21)

@@ -67,7 +67,8 @@ def test_pool_and_upsample_definitions():
     assert np.isclose((p * g).sum(), (x * unet.pool_backward(g)).sum())
 
 
-def test_network_backward_matches_finite_differences():
+@pytest.mark.parametrize("kind", [0, 1])
+def test_network_backward_matches_finite_differences(kind):
     rng = np.random.default_rng(6)
     n = 8
     total = sum(27 * ci * co + co for ci, co, *_ in unet.LAYERS)
@@ -82,7 +83,7 @@ def test_network_backward_matches_finite_differences():
     x0[2:5, 2:6, 3:5, 1] = 1.0
     target = rng.standard_normal((n, n, n))
     valid = (7, 7, 7)
-    loss, grad = unet.loss_grad(x0, params, target, 1.3, valid)
+    loss, grad = unet.loss_grad(x0, params, target, 1.3, valid, kind=kind)
     idx = rng.choice(total, 24, replace=False)
     # include parameters of the first, a skip-fed and the output layer
     idx = np.concatenate([idx, [5, total - 3, 27 * 8 * 16 + 16 + 100]])
@@ -91,5 +92,22 @@ def test_network_backward_matches_finite_differences():
         p1, p2 = params.copy(), params.copy()
         p1[i] += h
         p2[i] -= h
-        fd = (unet.loss_grad(x0, p1, target, 1.3, valid)[0] - unet.loss_grad(x0, p2, target, 1.3, valid)[0]) / (2 * h)
+        fd = (unet.loss_grad(x0, p1, target, 1.3, valid, kind=kind)[0] -
+              unet.loss_grad(x0, p2, target, 1.3, valid, kind=kind)[0]) / (2 * h)
         assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-7, (i, fd, grad[i])
+
+
+def test_laplace7_is_the_poisson_operator():
+    """laplace7 against the 7-point operator written out, and its symmetry."""
+    rng = np.random.default_rng(8)
+    u = rng.standard_normal((5, 6, 7))
+    m = np.zeros(u.shape, bool)
+    m[:4, :5, :6] = True
+    L = unet.laplace7(u, m)
+    i, j, k = 2, 3, 4
+    want = 6 * u[i, j, k] - (u[i - 1, j, k] + u[i + 1, j, k] + u[i, j - 1, k] + u[i, j + 1, k] + u[i, j, k - 1] +
+                             u[i, j, k + 1])
+    assert np.isclose(L[i, j, k], want)
+    assert L[3, 4, 5] == 6 * u[3, 4, 5] - (u[2, 4, 5] + u[3, 3, 5] + u[3, 4, 4])  # neighbours outside m count as 0
+    v = rng.standard_normal(u.shape)
+    assert np.isclose((unet.laplace7(u, m) * np.where(m, v, 0)).sum(), (np.where(m, u, 0) * unet.laplace7(v, m)).sum())
