@@ -1,11 +1,12 @@
 """Benchmark of the MI355X ES PIC step (one GPU).
 
-Default workload (configs C2): 128^3 cells, 64 particles per cell for each of
-electrons and ions (2 x 133M superparticles, FP64), a floating 16^3-cell cube
-conductor handled by the capacitance matrix, upstream x-face injection,
-multigrid Poisson to relative residual 1e-8 (two solves per step because of
-the conductor). Particles are drawn on the device (Philox Maxwellian,
-synthetic); injection is drawn on the host from the reference's streams.
+Default workload: 128^3 cells, 64 particles per cell for each of electrons
+and ions (2 x 133M superparticles, FP64), a floating 16^3-cell cube conductor
+handled by the capacitance matrix, upstream x-face injection, multigrid
+Poisson (FP32 V(2,2) cycles, FP64 defect correction) to relative residual
+1e-8 (two solves per step because of the conductor). Particles are drawn on
+the device (Philox Maxwellian, synthetic); injection is drawn on the host
+from the reference's streams. --workload c3 / c5 run the 256^3 FP32 cases.
 
 A step is one RootGrid::mainLoopES. `value` is particles/s of the push+deposit
 part of the step (gather, Boris, move, absorb, count, CIC deposit, re-binning
@@ -117,6 +118,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--ppc", type=int, default=0, help="override particles per cell (0 = config)")
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="",
+                    help="override the workload's particle precision (profiling the other kernel instance)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cmatrix-cache", default="")
     ap.add_argument("--guess-weights", default=os.path.join(ROOT, "fopis_amd", "weights", "guess_c4.npz"),
@@ -146,8 +149,24 @@ def main():
     wl = WORKLOADS[args.workload]
     log(F.device_info())
     t0 = time.perf_counter()
-    h = Host(os.path.join(ROOT, "fopis_amd", "configs", wl["json"]), device_loader=True, device_seed=20251202,
+    cfg_path = os.path.join(ROOT, "fopis_amd", "configs", wl["json"])
+    tmp_cfg = None
+    if args.precision:
+        import tempfile
+
+        with open(cfg_path) as f:
+            cfg = json.load(f)
+        cfg.setdefault("Build", {})["precision"] = args.precision
+        for o in cfg.get("Object", {}).values():
+            o["file_name"] = os.path.join(os.path.dirname(cfg_path), o["file_name"])
+        fd, tmp_cfg = tempfile.mkstemp(suffix=".json")
+        with os.fdopen(fd, "w") as f:
+            json.dump(cfg, f)
+        cfg_path = tmp_cfg
+    h = Host(cfg_path, device_loader=True, device_seed=20251202,
              ppc_override=args.ppc, build_cmatrix=True, initialize=True, cmatrix_cache=args.cmatrix_cache)
+    if tmp_cfg:
+        os.unlink(tmp_cfg)
     ctx = h.ctx
     nparts = sum(ctx.num_particles(s) for s in range(h.S))
     log(f"setup {time.perf_counter() - t0:.1f} s, {nparts} particles")
@@ -194,7 +213,9 @@ def main():
         "dtype": "f64" if h.desc.precision == 0 else "f32",
         "data": "synthetic (device Philox Maxwellian load; host mt19937 injection)",
         "config": {"workload": wl["name"], "particles": int(n_avg), "ppc": args.ppc or None,
-                   "poisson_tol": 1e-8, "mg": "V(2,2) red-black SOR omega=1.5, FP64 defect correction"},
+                   "poisson_tol": 1e-8,
+                   "mg": "V(2,2) red-black SOR omega=1.5 in %s, FP64 defect correction"
+                         % ("FP32" if h.desc.mg_precision == 1 else "FP64")},
         "push_deposit_ms": part_ms,
         "tile_kernel_ms": tile_ms,
         "bin_ms": bin_ms,
@@ -211,7 +232,7 @@ def main():
                      "kernel": "k_tile (gather+Boris+move+absorb+count+deposit+bin), all species per step",
                      "algorithmic_bytes_per_particle": bpp},
     }
-    tr = pmc_traffic(args.workload)
+    tr = pmc_traffic(args.workload) if not args.precision else None
     if tr:
         # HBM bytes of one k_tile launch (one species) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
         line["roofline"]["traffic"] = tr.get("bytes_per_launch")

@@ -120,12 +120,18 @@ __global__ void k_colsum(const double* C, int n, double* colsum) {
   colsum[j] = s;
 }
 
-__global__ void k_total(const double* C, int64_t nn, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double r = 0.0;
-    for (int64_t i = 0; i < nn; ++i) r += C[i];
-    *out = r;
+// Sum of all elements of C, as the sum of its column sums (one block, fixed tree).
+__global__ __launch_bounds__(256) void k_total(const double* colsum, int n, double* out) {
+  __shared__ double s[256];
+  double r = 0.0;
+  for (int j = threadIdx.x; j < n; j += 256) r += colsum[j];
+  s[threadIdx.x] = r;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) *out = s[0];
 }
 
 // sumWholeCharge: charge_map += tmp; tmp = 0
@@ -242,7 +248,7 @@ void objects_set(Ctx& c, int o, const uint8_t* mask, int ncmat, const int32_t* i
 static void finish_cmatrix(Ctx& c, ObjectDev& ob) {
   const int n = ob.ncmat;
   k_colsum<<<div_up(n, 256), 256, 0, c.stream>>>(ob.C.p, n, ob.colsum.p);
-  k_total<<<1, 1, 0, c.stream>>>(ob.C.p, (int64_t)n * n, ob.scal.p + 3);
+  k_total<<<1, 256, 0, c.stream>>>(ob.colsum.p, n, ob.scal.p + 3);
   FOPIS_HIP(hipMemcpyAsync(&ob.total_cmat, ob.scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   FOPIS_HIP(hipStreamSynchronize(c.stream));
 }

@@ -209,3 +209,47 @@ def conv3d_backward(x_bf16, w, gy_bf16, up=False, want_dx=True):
                                          _p(gy, C.c_uint16), _p(dw, C.c_float), _p(db, C.c_float),
                                          _p(dx, C.c_uint16)))
     return dw, db, dx
+
+
+def bench_layers(n: int = 128, reps: int = 20, seed: int = 1):
+    """Per-layer time of the network's convolutions at an n^3 grid (random BF16
+    input and weights), and of one whole guess (apply) on an (n+1)^3-node context."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for name, (ci, co, lvl, up, relu) in zip(NAMES, LAYERS):
+        s = n >> lvl
+        xs = (s // 2,) * 3 if up else (s,) * 3
+        x = to_bf16(rng.uniform(-1, 1, xs + (ci,)).astype(np.float32))
+        w = (rng.standard_normal((27, ci, co)) * 0.05).astype(np.float32)
+        _, ms = conv3d(x, w, np.zeros(co, np.float32), up=bool(up), relu=bool(relu), reps=reps)
+        flop = 2.0 * 27 * ci * co * s ** 3
+        byt = 2.0 * (ci * (s // 2 if up else s) ** 3 + co * s ** 3 * (2 if name in ("d2a", "d1a", "d0a") else 1))
+        out.append({"layer": name, "cin": ci, "cout": co, "grid": s, "up": bool(up), "us": ms * 1e3,
+                    "tflops": flop / ms / 1e9, "gbs": byt / ms / 1e6})
+    nn = n + 1
+    ctx = F.Context(F.make_desc(nn, nn, nn, [(-1.0, 1.0, 1.0)], eps0=1.0, mg_precision=F.FP32))
+    g = Guess(ctx, seed=seed)
+    rho = np.zeros(ctx.field_shape(F.RHO))
+    rho[2:nn, 2:nn, 2:nn] = rng.standard_normal((nn - 2,) * 3)
+    ctx.set_field(F.RHO, rho)
+    for _ in range(3):
+        g.apply()
+    ctx.synchronize()
+    ctx.reset_timers()
+    for _ in range(reps):
+        g.apply()
+    ctx.synchronize()
+    apply_ms = ctx.phase_time("guess")[0]
+    ctx.close()
+    return out, apply_ms
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+
+    n = int(sys.argv[1]) if len(sys.argv) > 1 else 128
+    layers, apply_ms = bench_layers(n)
+    for r in layers:
+        print(json.dumps(r))
+    print(json.dumps({"apply_ms": apply_ms, "grid": n}))
