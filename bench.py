@@ -5,8 +5,10 @@ and ions (2 x 133M superparticles, FP64), a floating 16^3-cell cube conductor
 handled by the capacitance matrix, upstream x-face injection, multigrid
 Poisson (FP32 V(2,2) cycles, FP64 defect correction) to relative residual
 1e-8 (two solves per step because of the conductor). Particles are drawn on
-the device (Philox Maxwellian, synthetic); injection is drawn on the host
-from the reference's streams. --workload c3 / c5 run the 256^3 FP32 cases.
+the device (Philox Maxwellian, synthetic); boundary injection keeps the
+reference's per-face counts and residues on the host and draws the particles
+on the device (--injection host draws them on the host from the reference's
+mt19937 streams instead). --workload c3 / c5 run the 256^3 FP32 cases.
 
 A step is one RootGrid::mainLoopES. `value` is particles/s of the push+deposit
 part of the step (gather, Boris, move, absorb, count, CIC deposit, re-binning
@@ -120,6 +122,8 @@ def main():
     ap.add_argument("--ppc", type=int, default=0, help="override particles per cell (0 = config)")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="",
                     help="override the workload's particle precision (profiling the other kernel instance)")
+    ap.add_argument("--injection", choices=["device", "host"], default="device",
+                    help="boundary injection drawn on the device (Philox, same law) or on the host (mt19937)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cmatrix-cache", default="")
     ap.add_argument("--guess-weights", default=os.path.join(ROOT, "fopis_amd", "weights", "guess_c4.npz"),
@@ -164,7 +168,8 @@ def main():
             json.dump(cfg, f)
         cfg_path = tmp_cfg
     h = Host(cfg_path, device_loader=True, device_seed=20251202,
-             ppc_override=args.ppc, build_cmatrix=True, initialize=True, cmatrix_cache=args.cmatrix_cache)
+             ppc_override=args.ppc, build_cmatrix=True, initialize=True, cmatrix_cache=args.cmatrix_cache,
+             device_injection=args.injection == "device")
     if tmp_cfg:
         os.unlink(tmp_cfg)
     ctx = h.ctx
@@ -211,9 +216,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if h.desc.precision == 0 else "f32",
-        "data": "synthetic (device Philox Maxwellian load; host mt19937 injection)",
+        "data": "synthetic (device Philox Maxwellian load; %s)"
+                % ("device Philox boundary injection" if args.injection == "device" else "host mt19937 injection"),
         "config": {"workload": wl["name"], "particles": int(n_avg), "ppc": args.ppc or None,
-                   "poisson_tol": 1e-8,
+                   "poisson_tol": 1e-8, "injection": args.injection,
                    "mg": "V(2,2) red-black SOR omega=1.5 in %s, FP64 defect correction"
                          % ("FP32" if h.desc.mg_precision == 1 else "FP64")},
         "push_deposit_ms": part_ms,

@@ -45,6 +45,9 @@ def _compile(src: str) -> str:
     cmd = [hipcc(), *COMMON]
     if src.endswith(".cpp") and src != "host.cpp":
         cmd += ["-x", "hip"]
+    if src == "host.cpp":
+        # the host draws keep the reference's arithmetic: every product and sum rounded separately
+        cmd += ["-ffp-contract=off"]
     cmd += ["-c", os.path.join(SRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

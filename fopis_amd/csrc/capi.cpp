@@ -302,6 +302,7 @@ int fopis_main_loop_es(fopis_ctx* ctx) {
   NEED_CTX
   return guarded([&] {
     Ctx& c = ctx->c;
+    objects_reset_step(c);
     update_rho(c, true);
     double rr = 0.0;
     if (c.nobj == 0) {
@@ -456,6 +457,39 @@ int fopis_first_solve_cycles(fopis_ctx* ctx, int64_t* total_cycles) {
     return FOPIS_E_ARG;
   }
   return guarded([&] { *total_cycles = ctx->c.mg_first_cycles_total; });
+}
+
+int fopis_inject_maxwellian(fopis_ctx* ctx, int s, const int64_t* count6, uint64_t seed, const double* drift,
+                            double vth, double sigma) {
+  NEED_CTX
+  if (!count6 || !drift || s < 0 || s >= ctx->c.S) {
+    g_last_error = "bad species or null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] { particles_inject(ctx->c, s, count6, seed, drift, vth, sigma); });
+}
+
+int fopis_emit_photoelectrons(fopis_ctx* ctx, int obj, int s, const double* shine, double amount, double vth,
+                              uint64_t seed) {
+  NEED_CTX
+  return guarded([&] {
+    Ctx& c = ctx->c;
+    if (obj < 0 || obj >= c.nobj || !c.obj[obj].defined) throw Error(FOPIS_E_ARG, "object not defined");
+    if (s < 0 || s >= c.S || !shine) throw Error(FOPIS_E_ARG, "bad species or null shine vector");
+    int axis = -1;
+    for (int a = 0; a < 3; ++a)
+      if (std::fabs(shine[a]) == 1.0) {
+        axis = a;
+        break;
+      }
+    if (axis < 0)
+      throw Error(FOPIS_E_ARG, "shine vector must be [+-1,0,0], [0,+-1,0] or [0,0,+-1]");
+    if (!(amount >= 0.0) || amount > 1e6) throw Error(FOPIS_E_ARG, "emission amount per node out of range");
+    if (!(vth > 0.0)) throw Error(FOPIS_E_ARG, "vth must be positive");
+    // the reference's loop `for (int n = 0; n < amount; ++n)` runs ceil(amount) times
+    const int per_node = (int)std::ceil(amount);
+    particles_emit(c, obj, s, axis, shine[axis] > 0.0 ? 1 : -1, per_node, vth, seed);
+  });
 }
 
 int fopis_guess_init(fopis_ctx* ctx, uint64_t seed) {

@@ -166,6 +166,8 @@ struct SpeciesBins {
   int64_t inbox_n = 0;
   PinSoa host_inbox;     // staged on the host (pinned) until the next deposit
   bool upload_pending = false;  // an upload from host_inbox may still be queued on the stream
+  Soa<double> dev_inbox; // particles drawn on the device (injection) until the next deposit
+  int64_t dev_n = 0;
   Soa<R> stray;          // particles that did not fit a bin this step
   int64_t stray_cap = 0;
   int64_t total_cap = 0;
@@ -186,9 +188,14 @@ struct ObjectDev {
   DBuf<unsigned long long> absorbed;  // [S] absorbed superparticles this step
   DBuf<double> scal;         // [4]: potential, total_charge, ctp, scratch
   std::vector<double> current;         // [S] host copy (Spacecraft::current)
-  std::vector<double> host_current_add;  // emission currents added by the host
+  std::vector<double> host_current_add;      // emission current counted in the current step
+  std::vector<double> host_current_pending;  // emission staged for the next step
   double potential = 0, total_charge = 0;
   std::vector<int32_t> cmat_ijk;       // host copy
+  std::vector<uint8_t> cells;          // host copy of this object's cell map, (nx+1)^3
+  int lit_key = -1;                    // (axis, sign) the lit-node list below was made for
+  int n_lit = 0;
+  DBuf<int32_t> lit_ijk;               // [n_lit][3] glue ijk of the cmat nodes on the lit face
 };
 
 // Per-phase event timer.
@@ -271,6 +278,8 @@ void particles_load_maxwellian(Ctx& c, int s, int64_t n, uint64_t seed, const do
 int64_t particles_count(Ctx& c, int s);
 void particles_get(Ctx& c, int s, int64_t cap, double* const* host6, int64_t* n);
 void particles_stage(Ctx& c, int s, int64_t n, const double* const* host6);
+void particles_inject(Ctx& c, int s, const int64_t* count, uint64_t seed, const double* drift, double vth,
+                      double sigma);
 void particles_velocity(Ctx& c);
 void particles_position(Ctx& c);
 void particles_density_only(Ctx& c);
@@ -299,6 +308,9 @@ void objects_redistribute(Ctx& c);
 void objects_add_charge(Ctx& c, int o, const double* dq, const double* dcur);
 void objects_state(Ctx& c, int o, double* pot, double* q, double* cur, double* cmap);
 void objects_set_charge_map(Ctx& c, int o, const double* cmap);
+int objects_lit_nodes(Ctx& c, int o, int axis, int sign, const int32_t** dev_ijk);
+void objects_reset_step(Ctx& c);
+void particles_emit(Ctx& c, int o, int s, int axis, int sign, int per_node, double vth, uint64_t seed);
 
 void guess_init(Ctx& c, uint64_t seed);
 void guess_free(Ctx& c);

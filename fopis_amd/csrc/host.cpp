@@ -366,6 +366,14 @@ struct Object {
 struct StepBatch {
   std::vector<std::vector<Particle6>> particles;  // per species
   std::vector<std::vector<double>> dq, dcur;       // per object: S*ncmat, S
+  std::vector<std::array<int64_t, 6>> inj_count;   // per species: particles the device draws per face
+  uint64_t inj_seed = 0;
+  struct Emit {
+    int o, s;
+    double amount, vth;
+    uint64_t seed;
+  };
+  std::vector<Emit> emit;  // photoemission the device draws
 };
 
 // ---------------------------------------------------------- the facade
@@ -389,6 +397,7 @@ class GpuRootGrid {
   void emitParticlesFromObjects();
   std::vector<Particle6> generateInjection(int s);
   void emitInto(StepBatch& b);
+  void emitHost(StepBatch& b);  // the reference's mt19937 draws on the host
   void injectInto(StepBatch& b);
   void stageBatch(StepBatch& b);
   StepBatch newBatch() const;
@@ -426,6 +435,8 @@ class GpuRootGrid {
   std::vector<Species> sp_;
   std::vector<Object> objs_;
   std::vector<std::array<double, 6>> inj_residual_;  // RootGridParticle.cpp:4 (function-static there)
+  uint64_t inj_calls_ = 0;                           // device injection: one seed per call
+  uint64_t emit_calls_ = 0;                          // device photoemission: one seed per call
   double shine_[3] = {0, 0, 0};
   fopis_host_options opt_{};
   fopis_ctx* ctx_ = nullptr;
@@ -792,6 +803,29 @@ StepBatch GpuRootGrid::newBatch() const {
 // Each species draws from its own engines, so species run on separate threads
 // and the draws stay in the reference's per-stream order.
 void GpuRootGrid::injectInto(StepBatch& b) {
+  if (opt_.device_injection) {
+    // the per-face counts and residues exactly as the reference keeps them;
+    // the particles themselves are drawn on the device when the batch is staged
+    b.inj_count.assign(sp_.size(), std::array<int64_t, 6>{0, 0, 0, 0, 0, 0});
+    b.inj_seed = opt_.device_seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(inj_calls_ + 1));
+    ++inj_calls_;
+    const int ns[3] = {nx_, ny_, nz_};
+    for (size_t sid = 0; sid < sp_.size(); ++sid) {
+      const Species& s = sp_[sid];
+      if (s.type != "ambient") continue;
+      const std::array<double, 6> flux = calc_flux(s, u_);
+      for (int face = 0; face < 6; ++face) {
+        const int axis = face / 2;
+        if (d_.boundary[face] == 'P') continue;
+        const int a1 = axis == 0 ? 2 : (axis == 1 ? 2 : 0), a2 = axis == 0 ? 1 : (axis == 1 ? 0 : 1);
+        const double nd = 1.0 * (ns[a1] - 1) * (ns[a2] - 1) * flux[face];
+        const int num = static_cast<int>(std::floor(nd + inj_residual_[sid][face]));
+        inj_residual_[sid][face] += nd - num;
+        b.inj_count[sid][face] = num;
+      }
+    }
+    return;
+  }
   std::vector<std::vector<Particle6>> out(sp_.size());
   std::vector<std::thread> th;
   for (size_t s = 0; s < sp_.size(); ++s)
@@ -802,8 +836,18 @@ void GpuRootGrid::injectInto(StepBatch& b) {
 
 void GpuRootGrid::stageBatch(StepBatch& b) {
   for (size_t s = 0; s < sp_.size(); ++s) stage((int)s, b.particles[s]);
-  for (size_t o = 0; o < objs_.size(); ++o)
-    if (!objs_[o].emit_species.empty()) check(fopis_add_object_charge(ctx_, (int)o, b.dq[o].data(), b.dcur[o].data()));
+  for (size_t s = 0; s < b.inj_count.size(); ++s) {
+    const Species& sp = sp_[s];
+    if (sp.type != "ambient") continue;
+    check(fopis_inject_maxwellian(ctx_, (int)s, b.inj_count[s].data(), b.inj_seed + 7919ull * s, sp.drift,
+                                  sp.thermal_velocity(u_), sp.deviation(u_)));
+  }
+  for (const StepBatch::Emit& e : b.emit)
+    check(fopis_emit_photoelectrons(ctx_, e.o, e.s, shine_, e.amount, e.vth, e.seed));
+  if (!opt_.device_injection)
+    for (size_t o = 0; o < objs_.size(); ++o)
+      if (!objs_[o].emit_species.empty())
+        check(fopis_add_object_charge(ctx_, (int)o, b.dq[o].data(), b.dcur[o].data()));
 }
 
 void GpuRootGrid::injectParticlesFromBoundary() {
@@ -824,6 +868,22 @@ void GpuRootGrid::emitParticlesFromObjects() {
 // particles then get this step's position update, as in the reference where
 // emission precedes updateParticlePosition.
 void GpuRootGrid::emitInto(StepBatch& b) {
+  if (opt_.device_injection) {
+    // the device draws the particles, their surface charge and the current
+    for (int o = 0; o < (int)objs_.size(); ++o)
+      for (int sid : objs_[o].emit_species) {
+        const Species& s = sp_[sid];
+        const double amount = (1.0 * 1.0) * (s.current_density / (std::fabs(s.charge) * s.size));
+        const uint64_t seed = (opt_.device_seed ^ (0xD1B54A32D192ED03ull * (emit_calls_ + 1))) + 1009ull * o + 31ull * sid;
+        b.emit.push_back({o, sid, amount, s.thermal_velocity(u_), seed});
+      }
+    ++emit_calls_;
+    return;
+  }
+  emitHost(b);
+}
+
+void GpuRootGrid::emitHost(StepBatch& b) {
   for (int o = 0; o < (int)objs_.size(); ++o) {
     Object& ob = objs_[o];
     if (ob.emit_species.empty()) continue;
@@ -1081,6 +1141,30 @@ int64_t fopis_host_generate_injection(fopis_host* h, int s, int64_t cap, double*
     for (int64_t q = 0; q < n && q < cap; ++q)
       for (int c = 0; c < 6; ++c)
         if (o[c]) o[c][q] = ps[q].v[c];
+  });
+  return rc == FOPIS_OK ? n : rc;
+}
+
+int64_t fopis_host_generate_emission(fopis_host* h, int s, int64_t cap, double* x, double* y, double* z, double* vx,
+                                     double* vy, double* vz, double* dq, double* dcurrent) {
+  if (!h) return FOPIS_E_ARG;
+  int64_t n = -1;
+  const int rc = host_guard([&] {
+    if (s < 0 || s >= h->g->desc().nspecies) throw std::invalid_argument("species out of range");
+    fopis::StepBatch b = h->g->newBatch();
+    h->g->emitHost(b);
+    const auto& ps = b.particles[s];
+    n = (int64_t)ps.size();
+    double* o[6] = {x, y, z, vx, vy, vz};
+    for (int64_t q = 0; q < n && q < cap; ++q)
+      for (int c = 0; c < 6; ++c)
+        if (o[c]) o[c][q] = ps[q].v[c];
+    if (h->g->nobjects() > 0) {
+      const int nc = h->g->object(0).ncmat;
+      if (dq)
+        for (int q = 0; q < nc; ++q) dq[q] = b.dq[0][(size_t)s * nc + q];
+      if (dcurrent) *dcurrent = b.dcur[0][s];
+    }
   });
   return rc == FOPIS_OK ? n : rc;
 }

@@ -7,6 +7,8 @@
 //   getTotalCharge             src/Spacecraft/Spacecraft.cpp:248-259
 //   perfect-conductor charge   src/Spacecraft/Spacecraft.cpp:708-758
 //   Cmatrix::total             include/Spacecraft/Cmatrix.hpp:36-44
+#include <algorithm>
+
 #include "fopis_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -240,9 +242,69 @@ void objects_set(Ctx& c, int o, const uint8_t* mask, int ncmat, const int32_t* i
   FOPIS_HIP(hipMemsetAsync(ob.scal.p, 0, ob.scal.n * sizeof(double), st));
   ob.current.assign(c.S, 0.0);
   ob.host_current_add.assign(c.S, 0.0);
+  ob.host_current_pending.assign(c.S, 0.0);
   ob.total_cmat = 0.0;
+  ob.cells.assign(mask, mask + nm);
+  ob.lit_key = -1;
+  ob.n_lit = 0;
   c.nobj = std::max(c.nobj, o + 1);
   FOPIS_HIP(hipStreamSynchronize(st));
+}
+
+// The cmat nodes that emit when light travels along +axis (sign > 0) or
+// -axis: Spacecraft::is{X,Y,Z}surfacePoint (Spacecraft.cpp:461-504). A node
+// whose cell and the cell behind it along the axis both lie in the cell map
+// emits if exactly the one facing the light is inside the object (surface
+// minus for sign > 0, plus otherwise); elsewhere it emits if the four nodes
+// of the face square it spans are all cmat nodes. The list is made once per
+// (axis, sign) on the host and kept on the device.
+int objects_lit_nodes(Ctx& c, int o, int axis, int sign, const int32_t** dev_ijk) {
+  ObjectDev& ob = c.obj[o];
+  const int key = 2 * axis + (sign > 0 ? 0 : 1);
+  if (ob.lit_key != key) {
+    const int n1 = c.nx + 1, n2 = c.ny + 1, n3 = c.nz + 1;  // cell map extent
+    auto in_map = [&](int i, int j, int k) { return i >= 0 && j >= 0 && k >= 0 && i < n1 && j < n2 && k < n3; };
+    auto inside = [&](int i, int j, int k) { return ob.cells[((size_t)i * n2 + j) * n3 + k] != 0; };
+    std::vector<int64_t> nodes(ob.ncmat);
+    for (int q = 0; q < ob.ncmat; ++q)
+      nodes[q] = (int64_t)c.id3(ob.cmat_ijk[3 * q], ob.cmat_ijk[3 * q + 1], ob.cmat_ijk[3 * q + 2]);
+    std::vector<int64_t> sorted = nodes;
+    std::sort(sorted.begin(), sorted.end());
+    auto is_cmat = [&](int i, int j, int k) {
+      if (i < 0 || j < 0 || k < 0 || i >= c.cx || j >= c.cy || k >= c.cz) return false;
+      return std::binary_search(sorted.begin(), sorted.end(), (int64_t)c.id3(i, j, k));
+    };
+    const int t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;  // the face's two tangential axes
+    std::vector<int32_t> lit;
+    for (int q = 0; q < ob.ncmat; ++q) {
+      int p[3] = {ob.cmat_ijk[3 * q], ob.cmat_ijk[3 * q + 1], ob.cmat_ijk[3 * q + 2]};
+      int b[3] = {p[0], p[1], p[2]};
+      b[axis] -= 1;
+      bool on;
+      if (in_map(p[0], p[1], p[2]) && in_map(b[0], b[1], b[2])) {
+        const bool here = inside(p[0], p[1], p[2]), behind = inside(b[0], b[1], b[2]);
+        on = sign > 0 ? (here && !behind) : (!here && behind);
+      } else {
+        int u[3] = {p[0], p[1], p[2]}, w[3] = {p[0], p[1], p[2]}, uw[3] = {p[0], p[1], p[2]};
+        u[t1] += 1;
+        w[t2] += 1;
+        uw[t1] += 1;
+        uw[t2] += 1;
+        on = is_cmat(p[0], p[1], p[2]) && is_cmat(u[0], u[1], u[2]) && is_cmat(w[0], w[1], w[2]) &&
+             is_cmat(uw[0], uw[1], uw[2]);
+      }
+      if (on) lit.insert(lit.end(), p, p + 3);
+    }
+    ob.n_lit = (int)(lit.size() / 3);
+    ob.lit_ijk.alloc(std::max<size_t>(lit.size(), 3));
+    if (!lit.empty())
+      FOPIS_HIP(hipMemcpyAsync(ob.lit_ijk.p, lit.data(), lit.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                               c.stream));
+    FOPIS_HIP(hipStreamSynchronize(c.stream));
+    ob.lit_key = key;
+  }
+  *dev_ijk = ob.lit_ijk.p;
+  return ob.n_lit;
 }
 
 static void finish_cmatrix(Ctx& c, ObjectDev& ob) {
@@ -303,6 +365,20 @@ void objects_get_cmatrix(Ctx& c, int o, double* C) {
   FOPIS_HIP(hipMemcpy(C, ob.C.p, n2 * sizeof(double), hipMemcpyDeviceToHost));
 }
 
+// RootGrid::resetObjects -> Spacecraft::resetCurrent at the top of mainLoopES
+// (RootGrid.cpp:82-86, RootGridObject.cpp:162-170, Spacecraft.cpp:303-307):
+// the current counts one step's captured and emitted superparticles. Emission
+// staged before the step belongs to it.
+void objects_reset_step(Ctx& c) {
+  for (int o = 0; o < c.nobj; ++o) {
+    ObjectDev& ob = c.obj[o];
+    if (!ob.defined) continue;
+    FOPIS_HIP(hipMemsetAsync(ob.absorbed.p, 0, c.S * sizeof(unsigned long long), c.stream));
+    ob.host_current_add = ob.host_current_pending;
+    ob.host_current_pending.assign(c.S, 0.0);
+  }
+}
+
 void objects_sum_whole_charge(Ctx& c) {
   for (int o = 0; o < c.nobj; ++o) {
     ObjectDev& ob = c.obj[o];
@@ -339,7 +415,7 @@ void objects_add_charge(Ctx& c, int o, const double* dq, const double* dcur) {
     FOPIS_HIP(hipMemcpy(ob.tmp_charge.p, cur.data(), n * sizeof(double), hipMemcpyHostToDevice));
   }
   if (dcur)
-    for (int s = 0; s < c.S; ++s) ob.host_current_add[s] += dcur[s];
+    for (int s = 0; s < c.S; ++s) ob.host_current_pending[s] += dcur[s];
 }
 
 void objects_state(Ctx& c, int o, double* pot, double* q, double* cur, double* cmap) {
@@ -353,7 +429,8 @@ void objects_state(Ctx& c, int o, double* pot, double* q, double* cur, double* c
     std::vector<unsigned long long> ab(c.S);
     FOPIS_HIP(hipMemcpy(ab.data(), ob.absorbed.p, c.S * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     for (int s = 0; s < c.S; ++s)
-      cur[s] = (double)ab[s] * (c.d.species[s].charge * c.d.species[s].size) + ob.host_current_add[s];
+      cur[s] = (double)ab[s] * (c.d.species[s].charge * c.d.species[s].size) + ob.host_current_add[s] +
+               ob.host_current_pending[s];
   }
   if (cmap && ob.ncmat > 0)
     FOPIS_HIP(hipMemcpy(cmap, ob.charge_map.p, (size_t)c.S * ob.ncmat * sizeof(double), hipMemcpyDeviceToHost));

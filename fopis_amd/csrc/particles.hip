@@ -986,6 +986,177 @@ __global__ void k_load_maxwellian(SoaPtr<R> f, int64_t n, uint64_t seed, double3
   f.c[5][i] = (R)(drift.z + sigma * (r2 * cos(t2)));
 }
 
+// Boundary injection (RootGrid::injectParticlesFromBoundary,
+// RootGridParticle.cpp:3-130, AmbientParticleType.cpp:101-157) drawn on the
+// device: particle i of this call belongs to the face whose range of the
+// prefix `first` holds it. Normal velocity: N(u_a, vth) conditioned on
+// pointing into the domain (the reference redraws until it does; here the
+// same truncated normal is drawn exactly by its inverse CDF); tangential
+// velocities N(u, sigma); position uniform over the face and, along the
+// normal, within v of it. Written as FP64 into the device staging.
+struct InjArgs {
+  int64_t first[7];      // prefix of the per-face counts (faces x-, x+, y-, y+, z-, z+)
+  double drift[3];
+  double vth, sigma;
+  double mx[3];          // n - 1 per axis
+  uint64_t seed;
+};
+
+__global__ void k_inject(SoaPtr<double> f, int64_t off, InjArgs A) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.first[6]) return;
+  int face = 0;
+  while (face < 5 && i >= A.first[face + 1]) ++face;
+  const int axis = face >> 1, side = face & 1;
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  U4 c{{(uint32_t)i, (uint32_t)(i >> 32), 0u, 0u}};
+  const U4 a = philox(c, k0, k1);
+  c.v[2] = 1u;
+  const U4 b = philox(c, k0, k1);
+  c.v[2] = 2u;
+  const U4 d = philox(c, k0, k1);
+  // normal component: truncated normal on the inward half-line
+  const double ua = A.drift[axis];
+  const double p0 = normcdf((0.0 - ua) / A.vth);  // P(v < 0)
+  const double uu = (u01(a.v[0], a.v[1]) + 0.5 / 9007199254740992.0);  // in (0, 1)
+  double pn = side == 0 ? p0 + uu * (1.0 - p0) : uu * p0;
+  pn = fmin(fmax(pn, 1e-300), 1.0 - 1.1102230246251565e-16);
+  double vn = ua + A.vth * normcdfinv(pn);
+  if (side == 0 && !(vn > 0.0)) vn = 1e-300;
+  if (side == 1 && !(vn < 0.0)) vn = -1e-300;
+  // tangential components (Box-Muller)
+  const double r = sqrt(-2.0 * log(1.0 - u01(a.v[2], a.v[3])));
+  const double th = 6.283185307179586 * u01(b.v[0], b.v[1]);
+  double v[3];
+  const int t1 = axis == 0 ? 1 : 0, t2 = axis == 2 ? 1 : 2;
+  v[axis] = vn;
+  v[t1] = A.drift[t1] + A.sigma * (r * cos(th));
+  v[t2] = A.drift[t2] + A.sigma * (r * sin(th));
+  // position
+  double x[3];
+  x[t1] = A.mx[t1] * u01(b.v[2], b.v[3]);
+  x[t2] = A.mx[t2] * u01(d.v[0], d.v[1]);
+  const double un = u01(d.v[2], d.v[3]);
+  x[axis] = side == 0 ? un * vn : A.mx[axis] + (1.0 - un) * vn;
+  const int64_t o = off + i;
+  f.c[0][o] = x[0];
+  f.c[1][o] = x[1];
+  f.c[2][o] = x[2];
+  f.c[3][o] = v[0];
+  f.c[4][o] = v[1];
+  f.c[5][o] = v[2];
+}
+
+// Photoemission from the lit face of an object (Spacecraft::emitParticles,
+// Spacecraft.cpp:829-900, photoelectron branch) drawn on the device. Thread i
+// emits particle i % per_node of lit node i / per_node:
+// - velocity (PhotoElectronParticleType.cpp:9-106): speed vth, polar angle
+//   to the outward normal asin(2u - 1), azimuth 2 pi u';
+// - birth point: the node plus a uniform offset in the face square, moved
+//   back along v by (1 - rt) of a step, rt uniform (inside the object);
+// - its charge leaves the surface at the point where the path crosses the
+//   face (Particle::getNext{X,Y,Z}CrossPoint, particle.cpp:206-255), shared
+//   bilinearly over the face square's four cmat nodes
+//   (subtractChargeOfParticleFrom{X,Y,Z}surface, Spacecraft.cpp:945-970);
+// - this step's position update then moves it by v (the reference emits
+//   between the velocity and the position update of mainLoopES), so the
+//   staged particle is at birth point + v; one that ends outside the domain
+//   is dropped when the staging is binned, as the position update's
+//   absorbing faces would drop it.
+struct EmitArgs {
+  int64_t n;        // lit nodes x per_node
+  int per_node, axis;
+  double ev;        // outward normal along axis: -shine[axis]
+  double vth, q;    // speed, superparticle charge
+  int cx, cy, cz;   // glue node extents
+  uint64_t seed;
+};
+
+__global__ void k_emit(SoaPtr<double> f, int64_t off, EmitArgs A, const int32_t* __restrict__ lit,
+                       const int32_t* __restrict__ node_cmat, double* dq) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const int64_t node = i / A.per_node;
+  const int ax = A.axis, t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+  const uint32_t k0 = (uint32_t)A.seed, k1 = (uint32_t)(A.seed >> 32);
+  U4 c{{(uint32_t)i, (uint32_t)(i >> 32), 0u, 0u}};
+  const U4 a = philox(c, k0, k1);
+  c.v[2] = 1u;
+  const U4 b = philox(c, k0, k1);
+  c.v[2] = 2u;
+  const U4 d = philox(c, k0, k1);
+  const double dep = asin(2.0 * u01(a.v[0], a.v[1]) - 1.0);
+  const double azi = 6.283185307179586 * u01(a.v[2], a.v[3]);
+  const double sd = sin(dep), ca = cos(azi), sa = sin(azi);
+  double v[3];
+  v[ax] = A.vth * cos(dep) * A.ev;
+  // tangential pair in the reference's assignment (y: cos, z: sin for an x face, and so on)
+  v[t1] = A.vth * sd * (ax == 1 ? sa : ca);
+  v[t2] = A.vth * sd * (ax == 1 ? ca : sa);
+  const double rt = u01(b.v[0], b.v[1]);
+  const double r[2] = {u01(b.v[2], b.v[3]), u01(d.v[0], d.v[1])};
+  const int ijk[3] = {lit[3 * node], lit[3 * node + 1], lit[3 * node + 2]};
+  double p[3];
+  for (int e = 0; e < 3; ++e) p[e] = (ijk[e] - 1.0) + (rt - 1.0) * v[e];
+  p[t1] += r[0];
+  p[t2] += r[1];
+  // crossing point with the face plane
+  double x[3] = {p[0], p[1], p[2]};
+  const double frac = p[ax] - floor(p[ax]);
+  if (v[ax] > 0.0) {
+    const double m = (1.0 - frac) / v[ax];
+    x[ax] = ceil(p[ax]);
+    x[t1] = p[t1] + v[t1] * m;
+    x[t2] = p[t2] + v[t2] * m;
+  } else if (v[ax] < 0.0) {
+    const double m = frac / (-v[ax]);
+    x[ax] = floor(p[ax]);
+    x[t1] = p[t1] + v[t1] * m;
+    x[t2] = p[t2] + v[t2] * m;
+  }
+  int ci[3];
+  double w1[3];
+  for (int e = 0; e < 3; ++e) {
+    ci[e] = (int)floor(x[e]) + 1;
+    w1[e] = x[e] - (ci[e] - 1);
+  }
+  const int ext[3] = {A.cx, A.cy, A.cz};
+#pragma unroll
+  for (int du = 0; du < 2; ++du)
+#pragma unroll
+    for (int dw = 0; dw < 2; ++dw) {
+      int nn[3] = {ci[0], ci[1], ci[2]};
+      nn[t1] += du;
+      nn[t2] += dw;
+      bool ok = true;
+      for (int e = 0; e < 3; ++e) ok = ok && nn[e] >= 0 && nn[e] < ext[e];
+      if (!ok) continue;
+      const int cm = node_cmat[((size_t)nn[0] * A.cy + nn[1]) * A.cz + nn[2]];
+      if (cm < 0) continue;
+      const double wu = du ? w1[t1] : 1.0 - w1[t1], ww = dw ? w1[t2] : 1.0 - w1[t2];
+      atomicAdd(dq + cm, -(A.q * wu * ww));
+    }
+  const int64_t o = off + i;
+  f.c[0][o] = p[0] + v[0];
+  f.c[1][o] = p[1] + v[1];
+  f.c[2][o] = p[2] + v[2];
+  f.c[3][o] = v[0];
+  f.c[4][o] = v[1];
+  f.c[5][o] = v[2];
+}
+
+// Number of particles in [0, m] per axis, one atomic per wavefront.
+__global__ void k_count_inside(SoaPtr<double> f, int64_t n, double mx, double my, double mz, unsigned long long* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool ok = false;
+  if (i < n) {
+    const double x = f.c[0][i], y = f.c[1][i], z = f.c[2][i];
+    ok = x >= 0.0 && y >= 0.0 && z >= 0.0 && x <= mx && y <= my && z <= mz;
+  }
+  const unsigned long long m = __ballot(ok);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, (unsigned long long)__popcll(m));
+}
+
 // ------------------------------------------------------------------ host side
 
 namespace {
@@ -994,6 +1165,7 @@ struct Scratch {
   RadixSorter sorter;
   DBuf<uint32_t> keys, perm, cnt, capa, ocapa;
   DBuf<int64_t> first, tmp;
+  DBuf<unsigned long long> ucount;
 };
 std::map<const Ctx*, Scratch> g_scratch;
 
@@ -1181,7 +1353,8 @@ void rebin(Ctx& c, int s) {
 template <class R>
 void prepare_inbox(Ctx& c, int s, bool consume) {
   auto& b = bins<R>(c)[s];
-  const int64_t n = (int64_t)b.host_inbox.n;
+  const int64_t nh = (int64_t)b.host_inbox.n;
+  const int64_t n = nh + b.dev_n;
   b.inbox_n = n;
   if (n == 0) return;
   hipStream_t st = c.stream;
@@ -1189,9 +1362,16 @@ void prepare_inbox(Ctx& c, int s, bool consume) {
   const int nt = c.ntiles;
   Soa<double>& raw = b.inbox_raw;
   raw.reserve(n);
-  for (int q = 0; q < 6; ++q)
-    FOPIS_HIP(hipMemcpyAsync(raw.c[q].p, b.host_inbox.c[q], n * sizeof(double), hipMemcpyHostToDevice, st));
-  b.upload_pending = true;
+  // host-staged particles first, then the ones drawn on the device
+  if (nh > 0) {
+    for (int q = 0; q < 6; ++q)
+      FOPIS_HIP(hipMemcpyAsync(raw.c[q].p, b.host_inbox.c[q], nh * sizeof(double), hipMemcpyHostToDevice, st));
+    b.upload_pending = true;
+  }
+  if (b.dev_n > 0)
+    for (int q = 0; q < 6; ++q)
+      FOPIS_HIP(hipMemcpyAsync(raw.c[q].p + nh, b.dev_inbox.c[q].p, b.dev_n * sizeof(double),
+                               hipMemcpyDeviceToDevice, st));
   TileArgs<double> a{};
   for (int q = 0; q < 3; ++q) {
     a.N[q] = c.Nc[q];
@@ -1219,7 +1399,10 @@ void prepare_inbox(Ctx& c, int s, bool consume) {
   FOPIS_HIP(hipGetLastError());
   // the pinned staging may be refilled once the upload has been consumed by the
   // stream (the step synchronises on its status read before the next stage)
-  if (consume) b.host_inbox.n = 0;
+  if (consume) {
+    b.host_inbox.n = 0;
+    b.dev_n = 0;
+  }
 }
 
 template <class R, int MODE>
@@ -1318,6 +1501,13 @@ void particles_init(Ctx& c) {
 template <class R>
 static void set_impl(Ctx& c, int s, int64_t n, const double* const* h) {
   hipStream_t st = c.stream;
+  // the species is replaced as a whole: particles staged for the next deposit go too
+  // (after any queued upload from the host staging has run)
+  auto& b = bins<R>(c)[s];
+  FOPIS_HIP(hipStreamSynchronize(st));
+  b.host_inbox.n = 0;
+  b.upload_pending = false;
+  b.dev_n = 0;
   Soa<R> flat;
   if (n > 0) {
     flat.alloc(n);
@@ -1368,6 +1558,22 @@ void particles_set(Ctx& c, int s, int64_t n, const double* const* h) {
     set_impl<double>(c, s, n, h);
 }
 
+// Device-staged particles inside the domain (binning drops the others).
+template <class R>
+static int64_t staged_inside(Ctx& c, SpeciesBins<R>& b) {
+  if (b.dev_n == 0) return 0;
+  Scratch& sc = scratch(c);
+  sc.ucount.alloc(1);
+  FOPIS_HIP(hipMemsetAsync(sc.ucount.p, 0, sizeof(unsigned long long), c.stream));
+  k_count_inside<<<div_up(b.dev_n, 256), 256, 0, c.stream>>>(ptrs(b.dev_inbox), b.dev_n, c.nx - 1.0, c.ny - 1.0,
+                                                              c.nz - 1.0, sc.ucount.p);
+  FOPIS_HIP(hipGetLastError());
+  unsigned long long k = 0;
+  FOPIS_HIP(hipMemcpyAsync(&k, sc.ucount.p, sizeof(k), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  return (int64_t)k;
+}
+
 template <class R>
 static int64_t count_impl(Ctx& c, int s) {
   auto& b = bins<R>(c)[s];
@@ -1376,24 +1582,50 @@ static int64_t count_impl(Ctx& c, int s) {
   FOPIS_HIP(hipStreamSynchronize(c.stream));
   int64_t n = 0;
   for (int v : cnt) n += v;
-  return n + (int64_t)b.host_inbox.n;
+  return n + (int64_t)b.host_inbox.n + staged_inside(c, b);
 }
 
 int64_t particles_count(Ctx& c, int s) { return c.fp32 ? count_impl<float>(c, s) : count_impl<double>(c, s); }
 
 template <class R>
 static void get_impl(Ctx& c, int s, int64_t cap, double* const* h, int64_t* nout) {
+  auto& b = bins<R>(c)[s];
   Soa<R> flat;
   const int64_t n = gather_all<R>(c, s, flat, false);
-  *nout = n;
-  if (n > cap) throw Error(FOPIS_E_ARG, "output buffer too small: need " + std::to_string(n));
-  if (n == 0) return;
+  // particles staged for the next step follow the resident ones: host-staged
+  // first, then the ones drawn on the device (the order they are binned in);
+  // staged particles outside the domain, which binning drops, are left out
+  const int64_t nh = (int64_t)b.host_inbox.n, nd = b.dev_n;
+  std::vector<double> dv(6 * (size_t)nd);
+  for (int q = 0; q < 6 && nd > 0; ++q)
+    FOPIS_HIP(hipMemcpyAsync(dv.data() + q * (size_t)nd, b.dev_inbox.c[q].p, nd * sizeof(double),
+                             hipMemcpyDeviceToHost, c.stream));
   Soa<double> out;
-  out.alloc(n);
-  k_to_f64<R><<<div_up(n, 256), 256, 0, c.stream>>>(ptrs(out), ptrs(flat), n);
-  for (int q = 0; q < 6; ++q)
+  if (n > 0) {
+    out.alloc(n);
+    k_to_f64<R><<<div_up(n, 256), 256, 0, c.stream>>>(ptrs(out), ptrs(flat), n);
+    FOPIS_HIP(hipGetLastError());
+  }
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  const double lim[3] = {c.nx - 1.0, c.ny - 1.0, c.nz - 1.0};
+  std::vector<int64_t> keep;
+  keep.reserve(nd);
+  for (int64_t i = 0; i < nd; ++i) {
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+      const double x = dv[a * (size_t)nd + i];
+      ok = ok && x >= 0.0 && x <= lim[a];
+    }
+    if (ok) keep.push_back(i);
+  }
+  *nout = n + nh + (int64_t)keep.size();
+  if (*nout > cap) throw Error(FOPIS_E_ARG, "output buffer too small: need " + std::to_string(*nout));
+  for (int q = 0; q < 6 && n > 0; ++q)
     FOPIS_HIP(hipMemcpyAsync(h[q], out.c[q].p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   FOPIS_HIP(hipStreamSynchronize(c.stream));
+  for (int q = 0; q < 6 && nh > 0; ++q) std::memcpy(h[q] + n, b.host_inbox.c[q], nh * sizeof(double));
+  for (int q = 0; q < 6; ++q)
+    for (size_t k = 0; k < keep.size(); ++k) h[q][n + nh + (int64_t)k] = dv[q * (size_t)nd + keep[k]];
 }
 
 void particles_get(Ctx& c, int s, int64_t cap, double* const* h, int64_t* n) {
@@ -1412,6 +1644,90 @@ void particles_stage(Ctx& c, int s, int64_t n, const double* const* h) {
     pending = false;
   }
   hb.append(h, (size_t)n, c.stream);
+}
+
+// Grows the device staging to hold n more particles, keeping what earlier
+// calls of this step drew.
+template <class R>
+static void grow_dev_inbox(Ctx& c, SpeciesBins<R>& b, int64_t n) {
+  if ((int64_t)b.dev_inbox.size() >= b.dev_n + n) return;
+  Soa<double> bigger;
+  bigger.alloc(std::max<int64_t>(2 * (b.dev_n + n), 1 << 16));
+  for (int q = 0; q < 6 && b.dev_n > 0; ++q)
+    FOPIS_HIP(hipMemcpyAsync(bigger.c[q].p, b.dev_inbox.c[q].p, b.dev_n * sizeof(double), hipMemcpyDeviceToDevice,
+                             c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  for (int q = 0; q < 6; ++q) b.dev_inbox.c[q].swap(bigger.c[q]);
+}
+
+template <class R>
+static void inject_impl(Ctx& c, int s, const int64_t* count, uint64_t seed, const double* drift, double vth,
+                        double sigma) {
+  auto& b = bins<R>(c)[s];
+  InjArgs A{};
+  A.first[0] = 0;
+  for (int f = 0; f < 6; ++f) {
+    if (count[f] < 0) throw Error(FOPIS_E_ARG, "negative injection count");
+    A.first[f + 1] = A.first[f] + count[f];
+  }
+  const int64_t n = A.first[6];
+  if (n == 0) return;
+  if (!(vth > 0.0) || !(sigma > 0.0)) throw Error(FOPIS_E_ARG, "vth and sigma must be positive");
+  for (int a = 0; a < 3; ++a) A.drift[a] = drift[a];
+  A.vth = vth;
+  A.sigma = sigma;
+  A.mx[0] = c.nx - 1;
+  A.mx[1] = c.ny - 1;
+  A.mx[2] = c.nz - 1;
+  A.seed = seed;
+  grow_dev_inbox(c, b, n);
+  k_inject<<<div_up(n, 256), 256, 0, c.stream>>>(ptrs(b.dev_inbox), b.dev_n, A);
+  FOPIS_HIP(hipGetLastError());
+  b.dev_n += n;
+}
+
+void particles_inject(Ctx& c, int s, const int64_t* count, uint64_t seed, const double* drift, double vth,
+                      double sigma) {
+  if (c.fp32)
+    inject_impl<float>(c, s, count, seed, drift, vth, sigma);
+  else
+    inject_impl<double>(c, s, count, seed, drift, vth, sigma);
+}
+
+template <class R>
+static void emit_impl(Ctx& c, int o, int s, int axis, int sign, int per_node, double vth, uint64_t seed) {
+  ObjectDev& ob = c.obj[o];
+  const int32_t* lit = nullptr;
+  const int nlit = objects_lit_nodes(c, o, axis, sign, &lit);
+  const int64_t n = (int64_t)nlit * per_node;
+  const double q = c.d.species[s].charge * c.d.species[s].size;
+  // every emitted superparticle leaves the object (Spacecraft.cpp:951)
+  ob.host_current_pending[s] -= q * (double)n;
+  if (n == 0) return;
+  auto& b = bins<R>(c)[s];
+  EmitArgs A{};
+  A.n = n;
+  A.per_node = per_node;
+  A.axis = axis;
+  A.ev = sign > 0 ? -1.0 : 1.0;
+  A.vth = vth;
+  A.q = q;
+  A.cx = c.cx;
+  A.cy = c.cy;
+  A.cz = c.cz;
+  A.seed = seed;
+  grow_dev_inbox(c, b, n);
+  k_emit<<<div_up(n, 256), 256, 0, c.stream>>>(ptrs(b.dev_inbox), b.dev_n, A, lit, ob.node_cmat.p,
+                                               ob.tmp_charge.p + (size_t)s * ob.ncmat);
+  FOPIS_HIP(hipGetLastError());
+  b.dev_n += n;
+}
+
+void particles_emit(Ctx& c, int o, int s, int axis, int sign, int per_node, double vth, uint64_t seed) {
+  if (c.fp32)
+    emit_impl<float>(c, o, s, axis, sign, per_node, vth, seed);
+  else
+    emit_impl<double>(c, o, s, axis, sign, per_node, vth, seed);
 }
 
 void particles_velocity(Ctx& c) {

@@ -22,6 +22,7 @@ class HostOptions(C.Structure):
         ("host_threads", C.c_int32),
         ("cmatrix_cache", C.c_char * 512),
         ("no_device", C.c_int32),
+        ("device_injection", C.c_int32),
     ]
 
 
@@ -46,6 +47,8 @@ def _bind(L):
     L.fopis_host_generate_ms.restype = C.c_double
     L.fopis_host_generate_injection.argtypes = [vp, C.c_int, i64] + [dp] * 6
     L.fopis_host_generate_injection.restype = i64
+    L.fopis_host_generate_emission.argtypes = [vp, C.c_int, i64] + [dp] * 8
+    L.fopis_host_generate_emission.restype = i64
     L.fopis_host_generate_initial.argtypes = [vp, C.c_int, i64] + [dp] * 6
     L.fopis_host_generate_initial.restype = i64
     L.fopis_host_flux.argtypes = [vp, C.c_int, dp]
@@ -75,7 +78,7 @@ class Host:
 
     def __init__(self, json_path: str, load_particles=True, device_loader=False, device_seed=12345,
                  ppc_override=0, nx_override=0, build_cmatrix=True, initialize=True, cmatrix_cache="",
-                 no_device=False):
+                 no_device=False, device_injection=False):
         self.L = _bind(F.load())
         o = HostOptions()
         o.load_particles = int(load_particles)
@@ -87,6 +90,7 @@ class Host:
         o.initialize = int(initialize)
         o.cmatrix_cache = cmatrix_cache.encode()
         o.no_device = int(no_device)
+        o.device_injection = int(device_injection)
         h = C.c_void_p()
         rc = self.L.fopis_host_open(json_path.encode(), C.byref(o), C.byref(h))
         if rc != 0:
@@ -146,6 +150,24 @@ class Host:
         if n > cap:
             raise RuntimeError("injection buffer too small")
         return dict(zip(("x", "y", "z", "vx", "vy", "vz"), [a[:n] for a in arrs]))
+
+    def generate_emission(self, s: int):
+        """One step of photoemission (every emitting object and species; advances
+        the streams): species s's particles, object 0's charge change for s and
+        its current change."""
+        cap = 1 << 22
+        arrs = [np.empty(cap) for _ in range(6)]
+        try:
+            nc = max(1, self.object(0)["ncmat"])
+        except F.FopisError:
+            nc = 1
+        dq = np.zeros(nc)
+        dcur = np.zeros(1)
+        n = self._ok(self.L.fopis_host_generate_emission(self.h, s, cap, *[F._dp(a) for a in arrs], F._dp(dq),
+                                                         F._dp(dcur)))
+        if n > cap:
+            raise RuntimeError("emission buffer too small")
+        return dict(zip(("x", "y", "z", "vx", "vy", "vz"), [a[:n] for a in arrs])), dq, float(dcur[0])
 
     def generate_initial(self, s: int) -> dict:
         d = self.desc

@@ -31,7 +31,8 @@ EXPORTS = [
     "fopis_guess_set_mode", "fopis_guess_apply", "fopis_guess_dataset_alloc", "fopis_guess_dataset_store",
     "fopis_guess_dataset_get", "fopis_guess_dataset_set", "fopis_guess_grad", "fopis_guess_train",
     "fopis_guess_eval", "fopis_guess_synth_boxes", "fopis_nn_conv3d", "fopis_nn_conv3d_backward",
-    "fopis_first_solve_cycles", "fopis_moments", "fopis_guess_set_loss",
+    "fopis_first_solve_cycles", "fopis_moments", "fopis_guess_set_loss", "fopis_inject_maxwellian",
+    "fopis_emit_photoelectrons",
 ]
 
 
@@ -242,6 +243,22 @@ class Context:
     def set_field(self, which: int, arr: np.ndarray, s: int = 0) -> None:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.field_shape(which))
         self._ok(self.L.fopis_set_field(self.h, which, s, _dp(a)))
+
+    def inject_maxwellian(self, s: int, counts, seed: int, drift, vth: float, sigma: float) -> None:
+        """Boundary injection drawn on the device: counts[face] particles per face (x-, x+, y-, y+, z-, z+)."""
+        c6 = np.ascontiguousarray(counts, dtype=np.int64)
+        dr = np.ascontiguousarray(drift, dtype=np.float64)
+        self.L.fopis_inject_maxwellian.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64), C.c_uint64,
+                                                   C.POINTER(C.c_double), C.c_double, C.c_double]
+        self._ok(self.L.fopis_inject_maxwellian(self.h, s, c6.ctypes.data_as(C.POINTER(C.c_int64)), seed, _dp(dr),
+                                                vth, sigma))
+
+    def emit_photoelectrons(self, o: int, s: int, shine, amount: float, vth: float, seed: int) -> None:
+        """Photoemission from object o's lit face drawn on the device (species s, ceil(amount) per node)."""
+        sh = np.ascontiguousarray(shine, dtype=np.float64)
+        self.L.fopis_emit_photoelectrons.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_double,
+                                                     C.c_double, C.c_uint64]
+        self._ok(self.L.fopis_emit_photoelectrons(self.h, o, s, _dp(sh), amount, vth, seed))
 
     def moments(self, s: int) -> dict:
         """count, mean position, mean velocity and velocity variance of species s (FP64)."""

@@ -94,7 +94,8 @@ int fopis_device_info(char* buf, int buflen);
 
 /* Replaces the initial load's push_back into Grid::particles[s]
  * (RootGrid.cpp:57-79): n particles of species s, FP64 SoA, replace the
- * species' current set. Particles are tile-binned on the device. */
+ * species' current set, including particles staged for the next deposit.
+ * Particles are tile-binned on the device. */
 int fopis_set_particles(fopis_ctx* ctx, int s, int64_t n, const double* x, const double* y, const double* z,
                         const double* vx, const double* vy, const double* vz);
 /* Draws n particles of species s on the device and replaces the species'
@@ -106,10 +107,41 @@ int fopis_set_particles(fopis_ctx* ctx, int s, int64_t n, const double* x, const
  * AmbientParticleType.cpp:76-93) at sizes where drawing on the host is slow. */
 int fopis_load_maxwellian(fopis_ctx* ctx, int s, int64_t n, uint64_t seed, const double* lo, const double* hi,
                           const double* drift, double sigma);
+/* RootGrid::injectParticlesFromBoundary (RootGridParticle.cpp:3-130) with the
+ * particles drawn on the device: count6[face] particles (faces x-, x+, y-, y+,
+ * z-, z+; the caller keeps the reference's floor(dt*A*flux + residue) counts),
+ * normal velocity N(drift, vth) conditioned on pointing inward (the law of
+ * AmbientParticleType::generateNewInjectionVelocity, AmbientParticleType.cpp:
+ * 101-157, drawn exactly by its inverse CDF instead of by redrawing),
+ * tangential N(drift, sigma), position uniform over the face and within v of
+ * it. Counter-based (Philox4x32-10) stream keyed by seed; the particles join
+ * at the next deposit like fopis_add_particles. Statistically equivalent to
+ * the reference's mt19937 streams, not bit-identical. */
+int fopis_inject_maxwellian(fopis_ctx* ctx, int s, const int64_t* count6, uint64_t seed, const double* drift,
+                            double vth, double sigma);
+/* Photoemission drawn on the device: replaces Spacecraft::emitParticles'
+ * photoelectron branch (Spacecraft.cpp:829-900, called from
+ * RootGrid::emitParticlesFromObjects, RootGridObject.cpp:175-181). Every cmat
+ * node of object obj on the face lit by shine (an axis unit vector; the
+ * node rule of Spacecraft::is{X,Y,Z}surfacePoint, Spacecraft.cpp:461-504)
+ * emits ceil(amount) superparticles of species s (amount = dx^2 j / (|q| size),
+ * the reference's per-node count): speed vth, polar angle asin(2u - 1) to the
+ * outward normal (PhotoElectronParticleType.cpp:9-106), born inside the
+ * object within one step of the face. Each one's charge leaves the surface
+ * at its face crossing point, bilinearly over that face square's cmat nodes
+ * (Spacecraft.cpp:945-970), and counts in the object's current for the next
+ * step. The particles join the next deposit moved by v (the reference emits
+ * between the velocity and the position update); those that end outside
+ * the domain are dropped when binned. Philox stream keyed by seed:
+ * statistically equivalent to the reference's mt19937 streams. */
+int fopis_emit_photoelectrons(fopis_ctx* ctx, int obj, int s, const double* shine, double amount, double vth,
+                              uint64_t seed);
 /* Number of live particles of species s (Grid::getValidParticleNumber,
  * GridParticleTreatment.cpp:120-128). */
 int fopis_num_particles(fopis_ctx* ctx, int s, int64_t* n);
-/* Copies species s out in device order (tile-major). cap = buffer length. */
+/* Copies species s out in device order (tile-major), followed by the
+ * particles staged for the next deposit (added from the host, then drawn on
+ * the device). cap = buffer length. */
 int fopis_get_particles(fopis_ctx* ctx, int s, int64_t cap, double* x, double* y, double* z, double* vx,
                         double* vy, double* vz, int64_t* n);
 /* Stages new particles of species s (injected at the boundary or emitted by
@@ -182,7 +214,10 @@ int fopis_set_cmatrix(fopis_ctx* ctx, int obj, const double* in);
  * the object's temporary charge map, S*ncmat doubles, species-major. */
 int fopis_add_object_charge(fopis_ctx* ctx, int obj, const double* dq, const double* dcurrent);
 /* State after the last redistribution: potential, total charge, per-species
- * current (S), charge_map (S*ncmat, may be NULL). */
+ * current (S: the superparticle charge captured and emitted in the current
+ * step, reset at the start of each fopis_main_loop_es like
+ * Spacecraft::resetCurrent; emission staged before a step counts in it),
+ * charge_map (S*ncmat, may be NULL). */
 int fopis_get_object_state(fopis_ctx* ctx, int obj, double* potential, double* total_charge, double* current,
                            double* charge_map);
 /* Sets charge_map (S*ncmat), as initializeChargeMapOffset does (Spacecraft.cpp:160-183). */

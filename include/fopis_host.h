@@ -37,6 +37,9 @@ typedef struct fopis_host_options {
   int32_t host_threads;     /* threads for the host mt19937 streams (0 -> 6)               */
   char cmatrix_cache[512];  /* directory of cached C matrices ("" = none)                  */
   int32_t no_device;        /* 1: parse, normalise and read geometry only (no GPU needed)  */
+  int32_t device_injection; /* 1: new particles drawn on the device: boundary injection
+                               (counts and residues still as RootGridParticle.cpp:3-45)
+                               and photoemission (fopis_emit_photoelectrons)              */
 } fopis_host_options;
 
 /* Reads the JSON input and builds the grid, objects and particles. */
@@ -56,6 +59,13 @@ int fopis_host_reload(fopis_host* h, uint64_t seed);
  * (for parity of the host generator): returns the count, fills up to cap. */
 int64_t fopis_host_generate_injection(fopis_host* h, int s, int64_t cap, double* x, double* y, double* z, double* vx,
                                       double* vy, double* vz);
+/* Photoelectrons of species s the next emission would add, drawn on the host
+ * from the reference's streams without staging them (Spacecraft.cpp:829-900:
+ * positions after this step's move, those leaving the domain dropped), and
+ * object 0's surface charge change for s (ncmat doubles) and current change.
+ * For parity of the host emitter: returns the count, fills up to cap. */
+int64_t fopis_host_generate_emission(fopis_host* h, int s, int64_t cap, double* x, double* y, double* z, double* vx,
+                                     double* vy, double* vz, double* dq, double* dcurrent);
 /* The initial load of species s as the host draws it from the reference's
  * streams (RootGrid.cpp:57-79), without uploading: returns the count. */
 int64_t fopis_host_generate_initial(fopis_host* h, int s, int64_t cap, double* x, double* y, double* z, double* vx,

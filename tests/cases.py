@@ -42,9 +42,12 @@ def oracle_params(cfg: dict, nx: int | None = None, ppc: int | None = None, with
             fn = o["file_name"]
             if not os.path.isabs(fn):
                 fn = os.path.join(CONFIGS, fn)
+            order = amb + photo
+            emit = [order.index(k) for k in o.get("emit_particles", {}) if k in photo]
             objs.append({"file_name": fn, "is_potential_fixed": int(o.get("is_potential_fixed", False)),
                          "fixed_potential": o.get("fixed_potential", 0.0),
-                         "initial_potential_offset": o.get("initial_potential_offset", 0.0)})
+                         "initial_potential_offset": o.get("initial_potential_offset", 0.0),
+                         "emit_species": emit[0] if emit else -1})
     return refcpu.params(n, n, n, env["dx"], env["dt"], sps, env["boundary"], tuple(b), tuple(shine), objs,
                          post_loop)
 
