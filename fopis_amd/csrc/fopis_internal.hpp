@@ -238,6 +238,7 @@ struct Ctx {
   DBuf<double> red;                       // reduction scratch
   DBuf<int32_t> status;                   // [8] stray counts etc.
   DBuf<uint8_t> mask;                     // (nx+1)^3 conductor cell ids (0 none, o+1)
+  DBuf<uint8_t> tile_near;                // [ntiles] 1 if a conductor cell lies within one tile of the tile
   DBuf<double> cap_rec;                   // captured particles, [S][cap_cap][8]
   int64_t cap_cap = 0;
   bool have_mask = false;

@@ -191,6 +191,8 @@ __global__ __launch_bounds__(256) void k_obj_drho(const double* C, int n, const 
 
 }  // namespace
 
+static int fdiv_floor(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
 void objects_set(Ctx& c, int o, const uint8_t* mask, int ncmat, const int32_t* ijk, int fixed, double fixed_pot) {
   if (o < 0 || o >= FOPIS_MAX_OBJECTS) throw Error(FOPIS_E_ARG, "object index out of range");
   if (ncmat < 0) throw Error(FOPIS_E_ARG, "negative ncmat");
@@ -210,6 +212,28 @@ void objects_set(Ctx& c, int o, const uint8_t* mask, int ncmat, const int32_t* i
   c.mask.alloc(nm);
   FOPIS_HIP(hipMemcpy(c.mask.p, hm.data(), nm, hipMemcpyHostToDevice));
   c.have_mask = true;
+  // per tile: does a conductor cell lie in the tile or its 26 neighbours?
+  // (a particle of the tile that ends in a cell of that box needs no mask read
+  // when the answer is no)
+  {
+    const int m1 = c.nx + 1, m2 = c.ny + 1, m3 = c.nz + 1;  // mask extent; cell ci has mask index ci + 1
+    std::vector<uint8_t> near((size_t)c.ntiles, 0);
+    for (int mi = 0; mi < m1; ++mi)
+      for (int mj = 0; mj < m2; ++mj)
+        for (int mk = 0; mk < m3; ++mk) {
+          if (!hm[((size_t)mi * m2 + mj) * m3 + mk]) continue;
+          const int ci = mi - 1, cj = mj - 1, ck = mk - 1;
+          // tiles whose neighbourhood box [8(t-1), 8(t+2)) holds this cell
+          const int t0[3] = {fdiv_floor(ci, kTile), fdiv_floor(cj, kTile), fdiv_floor(ck, kTile)};
+          for (int a0 = t0[0] - 1; a0 <= t0[0] + 1; ++a0)
+            for (int a1 = t0[1] - 1; a1 <= t0[1] + 1; ++a1)
+              for (int a2 = t0[2] - 1; a2 <= t0[2] + 1; ++a2)
+                if (a0 >= 0 && a1 >= 0 && a2 >= 0 && a0 < c.nt[0] && a1 < c.nt[1] && a2 < c.nt[2])
+                  near[((size_t)a0 * c.nt[1] + a1) * c.nt[2] + a2] = 1;
+        }
+    c.tile_near.alloc(std::max(c.ntiles, 1));
+    FOPIS_HIP(hipMemcpy(c.tile_near.p, near.data(), near.size(), hipMemcpyHostToDevice));
+  }
   ob.ncmat = ncmat;
   ob.fixed = fixed != 0;
   ob.fixed_potential = fixed_pot;

@@ -56,6 +56,7 @@ struct TileArgs {
   double* rho_g;
   double* dens_g;
   const uint8_t* mask;
+  const uint8_t* tile_near;  // [ntiles] a conductor cell within one tile of the tile
   int nobj;
   ObjArgs obj[FOPIS_MAX_OBJECTS];
   double* cap_rec;  // captured particles: 8 doubles each (state, object)
@@ -413,6 +414,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     }
   }
   const int ntot = nreg + nin;
+  const bool near_obj = a.nobj > 0 && a.tile_near[t] != 0;
   int wptr = 0, optr = 0;
   const int ocap = ((MODE & M_BIN) != 0) ? a.ocapa[t] : 0;
   const int rcap = a.capa[t];
@@ -537,7 +539,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
 
       // ---- conductor capture, then CIC deposit
       if constexpr ((MODE & M_DEP) != 0) {
-        if (valid && a.nobj > 0) {
+        // a tile with no conductor cell within one tile of it reads no mask for
+        // particles that end within that box
+        const bool in_box = ci >= o0 - kTile && ci < o0 + 2 * kTile && cj >= o1 - kTile && cj < o1 + 2 * kTile &&
+                            ck >= o2 - kTile && ck < o2 + 2 * kTile;
+        if (valid && a.nobj > 0 && (near_obj || !in_box)) {
           const int mi = ci + 1, mj = cj + 1, mk = ck + 1;
           const uint8_t m = a.mask[((size_t)mi * (a.cy - 1) + mj) * (a.cz - 1) + mk];
           if (m != 0) {
@@ -1233,6 +1239,7 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.rho_g = c.rho.p + (size_t)(s + 1) * c.nnode;
   a.dens_g = c.density.p + (size_t)s * c.ncell_d;
   a.mask = c.mask.p;
+  a.tile_near = c.tile_near.p;
   a.nobj = c.have_mask ? c.nobj : 0;
   for (int o = 0; o < c.nobj; ++o) {
     a.obj[o].node_cmat = c.obj[o].node_cmat.p;

@@ -61,8 +61,11 @@ def test_cmatrix_device_build_is_inverse_of_exact_b():
     np.testing.assert_allclose(Cm, Cm.T, rtol=1e-8, atol=1e-8 * np.abs(Cx).max())
 
 
-def test_capture_charge_and_current():
-    g = _oracle()
+@pytest.mark.parametrize("nx", [17, 41])
+def test_capture_charge_and_current(nx):
+    """At nx = 41 most tiles lie more than a tile away from the cube and skip
+    the per-particle conductor lookup; capture must not change."""
+    g = _oracle(nx=nx)
     Cm = g.cmatrix(0)
     ctx = _ctx_with_object(g, Cm)
     # a strong random field drives particles into the cube
