@@ -68,7 +68,32 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_io(verbose)
     return LIB
+
+
+IO_LIB = os.path.join(HERE, "libfopis_io.so")
+HDF5_PREFIX = os.environ.get("FOPIS_HDF5_PREFIX", "/opt/conda")
+
+
+def build_io(verbose: bool = True) -> str | None:
+    """libfopis_io.so (resume and capacity-matrix files in HDF5) when the HDF5 C
+    library is present; the step itself does not need it."""
+    inc, lib = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print(f"HDF5 not found under {HDF5_PREFIX}: libfopis_io.so not built")
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           "-I", inc, os.path.join(SRC, "h5io.cpp"), "-L", lib, "-lhdf5", f"-Wl,-rpath,{lib}", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile h5io.cpp failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, IO_LIB)
+    if verbose:
+        print(f"built {IO_LIB}")
+    return IO_LIB
 
 
 if __name__ == "__main__":

@@ -19,8 +19,12 @@
 #include <thread>
 #include <vector>
 
+#include <dlfcn.h>
+#include <sys/stat.h>
+
 #include "../../include/fopis.h"
 #include "../../include/fopis_host.h"
+#include "../../include/fopis_io.h"
 
 namespace fopis {
 
@@ -211,6 +215,10 @@ struct Species {
   double current_density = 0;  // normalised
   std::mt19937 mt[6];           // x, y, z, vx, vy, vz
   std::mt19937_64 mt_azimuth, mt_depression;
+  // ParticleType::generated_counts (particle_type.hpp:38-55): one count per
+  // generated position (x, y, z) and velocity (vx, vy, vz) of an ambient
+  // species, saved with the resume data
+  uint64_t generated[6] = {0, 0, 0, 0, 0, 0};
 
   // ParticleType::setId (ParticleType.cpp:19-27), rank 0
   void set_id(int i) {
@@ -376,6 +384,113 @@ struct StepBatch {
   std::vector<Emit> emit;  // photoemission the device draws
 };
 
+// ------------------------------------------------------- HDF5 files
+// The resume and capacity-matrix files go through libfopis_io.so (HDF5),
+// loaded next to this library on first use, so that the step itself does not
+// depend on HDF5.
+struct IoApi {
+  decltype(&fopis_io_open) open = nullptr;
+  decltype(&fopis_io_close) close = nullptr;
+  decltype(&fopis_io_write) write = nullptr;
+  decltype(&fopis_io_exists) exists = nullptr;
+  decltype(&fopis_io_info) info = nullptr;
+  decltype(&fopis_io_read) read = nullptr;
+  decltype(&fopis_io_last_error) last_error = nullptr;
+  std::string why;
+};
+
+static void io_anchor() {}
+
+static const IoApi& io_api() {
+  static const IoApi api = [] {
+    IoApi a;
+    std::string dir = ".";
+    Dl_info di;
+    if (dladdr(reinterpret_cast<void*>(&io_anchor), &di) && di.dli_fname) {
+      const std::string self = di.dli_fname;
+      const size_t sl = self.rfind('/');
+      if (sl != std::string::npos) dir = self.substr(0, sl);
+    }
+    const std::string path = dir + "/libfopis_io.so";
+    void* h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+    if (!h) {
+      const char* e = dlerror();
+      a.why = path + ": " + (e ? e : "cannot load");
+      return a;
+    }
+    a.open = reinterpret_cast<decltype(a.open)>(dlsym(h, "fopis_io_open"));
+    a.close = reinterpret_cast<decltype(a.close)>(dlsym(h, "fopis_io_close"));
+    a.write = reinterpret_cast<decltype(a.write)>(dlsym(h, "fopis_io_write"));
+    a.exists = reinterpret_cast<decltype(a.exists)>(dlsym(h, "fopis_io_exists"));
+    a.info = reinterpret_cast<decltype(a.info)>(dlsym(h, "fopis_io_info"));
+    a.read = reinterpret_cast<decltype(a.read)>(dlsym(h, "fopis_io_read"));
+    a.last_error = reinterpret_cast<decltype(a.last_error)>(dlsym(h, "fopis_io_last_error"));
+    if (!a.open || !a.close || !a.write || !a.exists || !a.info || !a.read || !a.last_error) {
+      a.open = nullptr;
+      a.why = path + ": missing entry points";
+    }
+    return a;
+  }();
+  if (!api.open) throw std::runtime_error("HDF5 files need libfopis_io.so: " + api.why);
+  return api;
+}
+
+// One HDF5 file, closed at scope exit.
+class H5 {
+ public:
+  H5(const std::string& path, bool create) : io_(io_api()), path_(path) {
+    if (io_.open(path.c_str(), create ? 1 : 0, &f_) != 0) throw std::runtime_error(io_.last_error());
+  }
+  ~H5() {
+    if (f_) io_.close(f_);
+  }
+  H5(const H5&) = delete;
+  H5& operator=(const H5&) = delete;
+  bool exists(const std::string& name) const { return io_.exists(f_, name.c_str()) == 1; }
+  std::vector<int64_t> dims(const std::string& name) const {
+    int r = 0;
+    int64_t d[8];
+    if (io_.info(f_, name.c_str(), &r, d) != 0) throw std::runtime_error(path_ + ": " + io_.last_error());
+    return std::vector<int64_t>(d, d + r);
+  }
+  void write(const std::string& name, int type, std::vector<int64_t> dims, const void* data) {
+    if (io_.write(f_, name.c_str(), type, (int)dims.size(), dims.data(), data) != 0)
+      throw std::runtime_error(path_ + ": " + io_.last_error());
+  }
+  void read(const std::string& name, int type, int64_t count, void* data) const {
+    if (io_.read(f_, name.c_str(), type, count, data) != 0) throw std::runtime_error(path_ + ": " + io_.last_error());
+  }
+  double scalar(const std::string& name) const {
+    double v = 0;
+    read(name, FOPIS_IO_F64, 1, &v);
+    return v;
+  }
+
+ private:
+  const IoApi& io_;
+  std::string path_;
+  fopis_h5* f_ = nullptr;
+};
+
+static void make_dirs(const std::string& dir) {
+  std::string cur;
+  size_t b = 0;
+  while (b <= dir.size()) {
+    size_t e = dir.find('/', b);
+    if (e == std::string::npos) e = dir.size();
+    cur = dir.substr(0, e);
+    if (!cur.empty()) mkdir(cur.c_str(), 0755);
+    b = e + 1;
+  }
+}
+
+static bool file_exists(const std::string& p) { return (bool)std::ifstream(p); }
+
+static std::string base_name(const std::string& p) {
+  const size_t sl = p.rfind('/');
+  return sl == std::string::npos ? p : p.substr(sl + 1);
+}
+
 // ---------------------------------------------------------- the facade
 class GpuRootGrid {
  public:
@@ -395,6 +510,19 @@ class GpuRootGrid {
   void updateEfield() { check(fopis_update_efield(ctx_)); }
   void injectParticlesFromBoundary();
   void emitParticlesFromObjects();
+  // RootGrid::saveResumeData / loadResumeData (RootGridIO.cpp:25-290) and
+  // Environment::saveInfo / loadInfo (Environment.cpp:379-438) into dir
+  void saveResumeData(const std::string& dir);
+  void loadResumeData(const std::string& dir);
+  // Grid::initializeForLoad (Grid.cpp:38-50): updateRho (which, with objects,
+  // solves once for the redistribution), then the reference E from the
+  // loaded field (here derived from the loaded phi, which is put back)
+  void initializeForLoad() {
+    updateRho();
+    if (!resume_phi_.empty()) check(fopis_set_field(ctx_, FOPIS_PHI, 0, resume_phi_.data()));
+    resume_phi_.clear();
+    updateEfield();
+  }
   std::vector<Particle6> generateInjection(int s);
   void emitInto(StepBatch& b);
   void emitHost(StepBatch& b);  // the reference's mt19937 draws on the host
@@ -446,6 +574,9 @@ class GpuRootGrid {
   StepBatch next_;
   bool have_next_ = false;
   std::string base_dir_;
+  double env_dx_ = 0, env_dt_ = 0;  // Environment::dx, dt as read from the input (SI)
+  bool load_job_ = false;
+  std::vector<double> resume_phi_;  // phi read by loadResumeData, restored by initializeForLoad
 };
 
 // AmbientParticleType::calcFlux (AmbientParticleType.cpp:8-57): rectangle
@@ -488,9 +619,14 @@ GpuRootGrid::GpuRootGrid(const std::string& path, const fopis_host_options& opt)
       throw std::invalid_argument("one GPU holds one rank's whole sub-domain: proc_x = proc_y = proc_z = 1");
   if (env.has("solver_type") && env.at("solver_type").string() != "ES")
     throw std::invalid_argument("only the electrostatic step (solver_type \"ES\") runs on this path");
-  if (env.has("job_type") && env.at("job_type").string() != "new")
-    throw std::invalid_argument("unknown or unsupported job_type");
+  if (env.has("job_type")) {  // "new", or "load": continue from the resume files (Initialize.cpp:44-47,83-95)
+    const std::string jt = env.at("job_type").string();
+    if (jt != "new" && jt != "load") throw std::invalid_argument("unknown job_type: " + jt);
+    load_job_ = jt == "load";
+  }
   const double dx = env.at("dx").number(), dt = env.at("dt").number();
+  env_dx_ = dx;
+  env_dt_ = dt;
   u_.set(dx, dt);
   const std::string boundary = env.has("boundary") ? env.at("boundary").string() : "DDDDDD";
   if (boundary.size() != 6) throw std::invalid_argument("boundary must have 6 letters");
@@ -632,25 +768,36 @@ GpuRootGrid::GpuRootGrid(const std::string& path, const fopis_host_options& opt)
     bool loaded = false;
     std::string cache;
     if (opt.cmatrix_cache[0]) {
-      cache = std::string(opt.cmatrix_cache) + "/" + ob.name + "_" + std::to_string(nx_) + "_" +
-              std::to_string(ob.ncmat) + ".cmat";
-      std::ifstream in(cache, std::ios::binary);
-      if (in) {
-        std::vector<double> C((size_t)ob.ncmat * ob.ncmat);
-        in.read(reinterpret_cast<char*>(C.data()), C.size() * sizeof(double));
-        if (in.gcount() == (std::streamsize)(C.size() * sizeof(double))) {
-          check(fopis_set_cmatrix(ctx_, o, C.data()));
-          loaded = true;
+      // IO::loadCmatrixData (IO.cpp:59-120): <cache>/objects/<obj file name>.h5 is
+      // used when its capacity_matrix is ncmat x ncmat and its grid_spacing and
+      // time_spacing equal the input's dx and dt
+      cache = std::string(opt.cmatrix_cache) + "/objects/" + base_name(ob.file) + ".h5";
+      if (file_exists(cache)) {
+        H5 f(cache, false);
+        if (f.exists("capacity_matrix")) {
+          const std::vector<int64_t> d = f.dims("capacity_matrix");
+          const bool valid = d.size() == 2 && d[0] == ob.ncmat && d[1] == ob.ncmat &&
+                             f.scalar("grid_spacing") == env_dx_ && f.scalar("time_spacing") == env_dt_;
+          if (valid) {
+            std::vector<double> C((size_t)ob.ncmat * ob.ncmat);
+            f.read("capacity_matrix", FOPIS_IO_F64, (int64_t)C.size(), C.data());
+            check(fopis_set_cmatrix(ctx_, o, C.data()));
+            loaded = true;
+          }
         }
       }
     }
     if (!loaded && opt.build_cmatrix) {
       check(fopis_build_cmatrix(ctx_, o));
       if (!cache.empty()) {
+        // IO::writeCmatrixData (IO.cpp:28-57): element [i][j] = Spacecraft::getCmatValue(i, j)
         std::vector<double> C((size_t)ob.ncmat * ob.ncmat);
         check(fopis_get_cmatrix(ctx_, o, C.data()));
-        std::ofstream out(cache, std::ios::binary);
-        out.write(reinterpret_cast<const char*>(C.data()), C.size() * sizeof(double));
+        make_dirs(std::string(opt.cmatrix_cache) + "/objects");
+        H5 f(cache, true);
+        f.write("capacity_matrix", FOPIS_IO_F64, {ob.ncmat, ob.ncmat}, C.data());
+        f.write("grid_spacing", FOPIS_IO_F64, {}, &env_dx_);
+        f.write("time_spacing", FOPIS_IO_F64, {}, &env_dt_);
       }
     }
     // Spacecraft::initializeChargeMapOffset (Spacecraft.cpp:160-183)
@@ -666,7 +813,114 @@ GpuRootGrid::GpuRootGrid(const std::string& path, const fopis_host_options& opt)
     }
   }
   if (opt.load_particles) load_initial();
-  if (opt.initialize) initialize();
+  if (load_job_) {
+    // RootGrid::loadResumeData, then Grid::initializeForLoad (Grid.cpp:38-50):
+    // rho from the loaded particles, reference E from the loaded phi
+    loadResumeData(opt.resume_dir[0] ? std::string(opt.resume_dir) : base_dir_ + "/resume");
+    initializeForLoad();
+  } else if (opt.initialize) {
+    initialize();
+  }
+}
+
+void GpuRootGrid::saveResumeData(const std::string& dir) {
+  make_dirs(dir);
+  {  // Environment::saveInfo (Environment.cpp:421-438)
+    H5 f(dir + "/environment.h5", true);
+    f.write("dx", FOPIS_IO_F64, {}, &env_dx_);
+    f.write("dt", FOPIS_IO_F64, {}, &env_dt_);
+    const int32_t ts = timestep_;
+    f.write("timestep", FOPIS_IO_I32, {}, &ts);
+  }
+  // RootGrid::saveResumeData (RootGridIO.cpp:42-53); only valid particles exist on the device
+  H5 f(dir + "/grid_id_0.h5", true);
+  for (const Species& s : sp_) {  // saveResumeParticleData (:133-181)
+    int64_t n = 0;
+    check(fopis_num_particles(ctx_, s.id, &n));
+    std::vector<double> a[6];
+    for (auto& v : a) v.resize((size_t)n);
+    int64_t got = 0;
+    check(fopis_get_particles(ctx_, s.id, n, a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data(),
+                              a[5].data(), &got));
+    const double size = (double)got;  // createDataSet<double>("size", ...) of a size_t
+    f.write(s.name + "/size", FOPIS_IO_F64, {}, &size);
+    f.write(s.name + "/generated_counts", FOPIS_IO_U64, {6}, s.generated);
+    const char* comp[6] = {"x", "y", "z", "vx", "vy", "vz"};
+    for (int c = 0; c < 6; ++c) f.write(s.name + "/" + comp[c], FOPIS_IO_F64, {got}, a[c].data());
+  }
+  {  // saveResumeFieldData (:227-262): phi, the edge E, and B (never sized in ES mode)
+    const int64_t cx = nx_ + 2, cy = ny_ + 2, cz = nz_ + 2;
+    std::vector<double> buf((size_t)(cx * cy * cz));
+    check(fopis_get_field(ctx_, FOPIS_PHI, 0, buf.data()));
+    f.write("Potential/phi", FOPIS_IO_F64, {cx, cy, cz}, buf.data());
+    const std::vector<int64_t> ed[3] = {{cx - 1, cy, cz}, {cx, cy - 1, cz}, {cx, cy, cz - 1}};
+    const char* en[3] = {"Efield/ex", "Efield/ey", "Efield/ez"};
+    for (int q = 0; q < 3; ++q) {
+      check(fopis_get_field(ctx_, FOPIS_EX + q, 0, buf.data()));
+      f.write(en[q], FOPIS_IO_F64, ed[q], buf.data());
+    }
+    for (const char* b : {"Bfield/bx", "Bfield/by", "Bfield/bz"}) f.write(b, FOPIS_IO_F64, {0, 0, 0}, nullptr);
+  }
+  for (int o = 0; o < (int)objs_.size(); ++o) {  // saveResumeObjectData (:273-280)
+    const int nc = objs_[o].ncmat;
+    std::vector<double> cmap(sp_.size() * (size_t)nc);
+    check(fopis_get_object_state(ctx_, o, nullptr, nullptr, nullptr, cmap.data()));
+    f.write(objs_[o].name + "/charge_map", FOPIS_IO_F64, {(int64_t)sp_.size(), nc}, cmap.data());
+  }
+}
+
+void GpuRootGrid::loadResumeData(const std::string& dir) {
+  {  // Environment::loadInfo (Environment.cpp:379-419)
+    const std::string fn = dir + "/environment.h5";
+    if (!file_exists(fn)) throw std::invalid_argument("[ERROR] resume file_name " + fn + " does not exist.");
+    H5 f(fn, false);
+    if (f.scalar("dx") != env_dx_) throw std::invalid_argument("dx in the resume data does not match the input's dx");
+    if (f.scalar("dt") != env_dt_) throw std::invalid_argument("dt in the resume data does not match the input's dt");
+    int32_t ts = 0;
+    f.read("timestep", FOPIS_IO_I32, 1, &ts);
+    timestep_ = ts;
+  }
+  const std::string fn = dir + "/grid_id_0.h5";
+  if (!file_exists(fn)) return;  // RootGrid::loadResumeData reads only an existing file
+  H5 f(fn, false);
+  for (Species& s : sp_) {  // loadResumeParticleData (:55-131)
+    const int64_t n = (int64_t)f.scalar(s.name + "/size");
+    std::vector<double> a[6];
+    if (n > 0) {
+      // ParticleType::proceedGeneratedCounts (ParticleType.cpp:70-103): each
+      // engine skips (saved count - own count) outputs
+      uint64_t target[6];
+      f.read(s.name + "/generated_counts", FOPIS_IO_U64, 6, target);
+      for (int c = 0; c < 6; ++c) {
+        if (target[c] < s.generated[c])
+          throw std::invalid_argument("resume generated_counts are behind this run's streams");
+        s.mt[c].discard(target[c] - s.generated[c]);
+        s.generated[c] = target[c];
+      }
+      const char* comp[6] = {"x", "y", "z", "vx", "vy", "vz"};
+      for (int c = 0; c < 6; ++c) {
+        a[c].resize((size_t)n);
+        f.read(s.name + "/" + comp[c], FOPIS_IO_F64, n, a[c].data());
+      }
+    }
+    check(fopis_set_particles(ctx_, s.id, n, a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data(),
+                              a[5].data()));
+  }
+  {  // loadResumeFieldData (:183-225): phi; E and B are checked for presence (E is derived from phi here)
+    const int64_t cx = nx_ + 2, cy = ny_ + 2, cz = nz_ + 2;
+    std::vector<double> phi((size_t)(cx * cy * cz));
+    f.read("Potential/phi", FOPIS_IO_F64, (int64_t)phi.size(), phi.data());
+    check(fopis_set_field(ctx_, FOPIS_PHI, 0, phi.data()));
+    resume_phi_ = std::move(phi);
+    for (const char* e : {"Efield/ex", "Efield/ey", "Efield/ez"})
+      if (!f.exists(e)) throw std::invalid_argument(std::string("resume data lacks ") + e);
+  }
+  for (int o = 0; o < (int)objs_.size(); ++o) {  // loadResumeObjectData (:264-271)
+    const int nc = objs_[o].ncmat;
+    std::vector<double> cmap(sp_.size() * (size_t)nc);
+    f.read(objs_[o].name + "/charge_map", FOPIS_IO_F64, (int64_t)cmap.size(), cmap.data());
+    check(fopis_set_object_charge_map(ctx_, o, cmap.data()));
+  }
 }
 
 bool GpuRootGrid::inside_object(double x, double y, double z) const {
@@ -699,6 +953,7 @@ void GpuRootGrid::generateInitial(int sid, std::vector<double> (&a)[6]) {
   std::vector<std::thread> th;
   for (int c = 0; c < 6; ++c) th.emplace_back(gen, c);
   for (auto& t : th) t.join();
+  for (int c = 0; c < 6; ++c) s.generated[c] += total;  // AmbientParticleType.cpp:86,97
   // Spacecraft::removeInnerParticle: drop the particles born inside an object
   size_t w = 0;
   for (size_t n = 0; n < total; ++n) {
@@ -778,6 +1033,7 @@ std::vector<Particle6> GpuRootGrid::generateInjection(int sid) {
       for (int c = 0; c < 3; ++c) {
         std::uniform_real_distribution<> d(lo[c], hi[c]);
         p.v[c] = d(s.mt[c]);
+        ++s.generated[c];  // generateNewPosition counts; the injection velocity does not
       }
       p.v[3] = v[0];
       p.v[4] = v[1];
@@ -1108,6 +1364,19 @@ int fopis_host_close(fopis_host* h) {
 }
 
 fopis_ctx* fopis_host_ctx(fopis_host* h) { return h ? h->g->ctx() : nullptr; }
+
+int fopis_host_save_resume(fopis_host* h, const char* dir) {
+  if (!h || !dir) return FOPIS_E_ARG;
+  return host_guard([&] { h->g->saveResumeData(dir); });
+}
+
+int fopis_host_load_resume(fopis_host* h, const char* dir) {
+  if (!h || !dir) return FOPIS_E_ARG;
+  return host_guard([&] {
+    h->g->loadResumeData(dir);
+    h->g->initializeForLoad();
+  });
+}
 
 int fopis_host_desc(fopis_host* h, fopis_desc* out) {
   if (!h || !out) return FOPIS_E_ARG;

@@ -260,6 +260,8 @@ __global__ void k_capture(const double* rec, int n, int cx, int cy, int cz, doub
   absorb(cd, r[0], r[1], r[2], r[3], r[4], r[5]);
 }
 
+constexpr int kWaves = kBlock / 64;  // wavefronts per block
+
 // Ranks of two predicates over the block, in thread order.
 __device__ __forceinline__ void block_rank2(bool pa, bool pb, int& ra, int& rb, int& ta, int& tb, int* sh) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -267,15 +269,15 @@ __device__ __forceinline__ void block_rank2(bool pa, bool pb, int& ra, int& rb, 
   const unsigned long long ma = __ballot(pa), mb = __ballot(pb);
   if (lane == 0) {
     sh[w] = __popcll(ma);
-    sh[4 + w] = __popcll(mb);
+    sh[kWaves + w] = __popcll(mb);
   }
   __syncthreads();
   int oa = 0, ob = 0;
   ta = 0;
   tb = 0;
 #pragma unroll
-  for (int q = 0; q < kBlock / 64; ++q) {
-    const int ca = sh[q], cb = sh[4 + q];
+  for (int q = 0; q < kWaves; ++q) {
+    const int ca = sh[q], cb = sh[kWaves + q];
     if (q < w) {
       oa += ca;
       ob += cb;
@@ -298,14 +300,14 @@ __device__ __forceinline__ void block_rank4(bool a0, bool b0, bool a1, bool b1, 
   const unsigned long long m[4] = {__ballot(a0), __ballot(b0), __ballot(a1), __ballot(b1)};
   if (lane == 0)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) sh[4 * k + w] = __popcll(m[k]);
+    for (int k = 0; k < 4; ++k) sh[kWaves * k + w] = __popcll(m[k]);
   __syncthreads();
   int off[4] = {0, 0, 0, 0}, tot[4] = {0, 0, 0, 0};
 #pragma unroll
-  for (int q = 0; q < kBlock / 64; ++q)
+  for (int q = 0; q < kWaves; ++q)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int cnt = sh[4 * k + q];
+      const int cnt = sh[kWaves * k + q];
       if (q < w) off[k] += cnt;
       tot[k] += cnt;
     }
@@ -368,7 +370,7 @@ template <class R, int MODE>
 __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   using SM = TileSmem<R, MODE>;
   __shared__ __attribute__((aligned(16))) char smem[SM::bytes];
-  __shared__ int sh_rank[16];
+  __shared__ int sh_rank[4 * kWaves];
   R* sE = reinterpret_cast<R*>(smem);
   double* sR = reinterpret_cast<double*>(smem + SM::e_bytes);
   // the count window is its own LDS array so that its atomics stay LDS atomics (ds_add_u32)
@@ -698,7 +700,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
 // it, neighbour by neighbour in direction order, each in outbox order.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_arrive(TileArgs<R> a) {
-  __shared__ int sh_rank[8];
+  __shared__ int sh_rank[2 * kWaves];
   const int t = blockIdx.x;
   const int tz = t % a.T[2], ty = (t / a.T[2]) % a.T[1], tx = t / (a.T[2] * a.T[1]);
   int n = a.cnt[t];

@@ -23,6 +23,7 @@ class HostOptions(C.Structure):
         ("cmatrix_cache", C.c_char * 512),
         ("no_device", C.c_int32),
         ("device_injection", C.c_int32),
+        ("resume_dir", C.c_char * 512),
     ]
 
 
@@ -43,6 +44,8 @@ def _bind(L):
     L.fopis_host_main_loop_es.argtypes = [vp]
     L.fopis_host_timestep.argtypes = [vp]
     L.fopis_host_reload.argtypes = [vp, C.c_uint64]
+    L.fopis_host_save_resume.argtypes = [vp, C.c_char_p]
+    L.fopis_host_load_resume.argtypes = [vp, C.c_char_p]
     L.fopis_host_generate_ms.argtypes = [vp]
     L.fopis_host_generate_ms.restype = C.c_double
     L.fopis_host_generate_injection.argtypes = [vp, C.c_int, i64] + [dp] * 6
@@ -78,7 +81,7 @@ class Host:
 
     def __init__(self, json_path: str, load_particles=True, device_loader=False, device_seed=12345,
                  ppc_override=0, nx_override=0, build_cmatrix=True, initialize=True, cmatrix_cache="",
-                 no_device=False, device_injection=False):
+                 no_device=False, device_injection=False, resume_dir=""):
         self.L = _bind(F.load())
         o = HostOptions()
         o.load_particles = int(load_particles)
@@ -91,6 +94,7 @@ class Host:
         o.cmatrix_cache = cmatrix_cache.encode()
         o.no_device = int(no_device)
         o.device_injection = int(device_injection)
+        o.resume_dir = resume_dir.encode()
         h = C.c_void_p()
         rc = self.L.fopis_host_open(json_path.encode(), C.byref(o), C.byref(h))
         if rc != 0:
@@ -129,6 +133,14 @@ class Host:
 
     def timestep(self) -> int:
         return self.L.fopis_host_timestep(self.h)
+
+    def save_resume(self, directory: str) -> None:
+        """RootGrid::saveResumeData + Environment::saveInfo into directory (HDF5)."""
+        self._ok(self.L.fopis_host_save_resume(self.h, directory.encode()))
+
+    def load_resume(self, directory: str) -> None:
+        """Environment::loadInfo + RootGrid::loadResumeData + initializeForLoad from directory."""
+        self._ok(self.L.fopis_host_load_resume(self.h, directory.encode()))
 
     def reload(self, seed: int) -> None:
         """Redraw the initial particle load on the device with another seed."""

@@ -40,11 +40,24 @@ typedef struct fopis_host_options {
   int32_t device_injection; /* 1: new particles drawn on the device: boundary injection
                                (counts and residues still as RootGridParticle.cpp:3-45)
                                and photoemission (fopis_emit_photoelectrons)              */
+  char resume_dir[512];     /* job_type "load": resume files directory ("" = <input dir>/resume) */
 } fopis_host_options;
 
 /* Reads the JSON input and builds the grid, objects and particles. */
 int fopis_host_open(const char* json_path, const fopis_host_options* opt, fopis_host** out);
 int fopis_host_close(fopis_host* h);
+/* RootGrid::saveResumeData with Environment::saveInfo (RootGridIO.cpp:42-53,
+ * Environment.cpp:421-438): writes dir/environment.h5 {dx, dt, timestep} and
+ * dir/grid_id_0.h5 {<species>/{size, generated_counts, x, y, z, vx, vy, vz},
+ * Potential/phi, Efield/{ex, ey, ez}, Bfield/{bx, by, bz}, <object>/charge_map}
+ * in the reference's names, types and shapes. Needs libfopis_io.so (HDF5). */
+int fopis_host_save_resume(fopis_host* h, const char* dir);
+/* Environment::loadInfo + RootGrid::loadResumeData + Grid::initializeForLoad
+ * (Environment.cpp:379-419, RootGridIO.cpp:25-40,55-131,183-225,264-271,
+ * Grid.cpp:38-50): dx and dt must match the input; particles, the random
+ * streams' generated counts, phi and the objects' charge maps are restored,
+ * then rho and the reference E are recomputed. */
+int fopis_host_load_resume(fopis_host* h, const char* dir);
 /* The device context underneath (owned by the facade). */
 fopis_ctx* fopis_host_ctx(fopis_host* h);
 /* Normalised description the facade built (species order = reference ids). */
