@@ -697,32 +697,66 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
 }
 
 // Arrivals: tile t appends the outbox entries its 26 neighbours addressed to
-// it, neighbour by neighbour in direction order, each in outbox order.
+// it, neighbour by neighbour in direction order, each in outbox order. Each
+// wavefront owns every kWaves-th direction: it first counts its matches, one
+// scan over the 27 counts gives every direction its offset, then it copies
+// its matches in outbox order (ranks by ballot) — two block barriers in all.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_arrive(TileArgs<R> a) {
-  __shared__ int sh_rank[2 * kWaves];
+  __shared__ int s_cnt[27];
+  __shared__ int s_off[28];
   const int t = blockIdx.x;
   const int tz = t % a.T[2], ty = (t / a.T[2]) % a.T[1], tx = t / (a.T[2] * a.T[1]);
-  int n = a.cnt[t];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  // the source tile of direction d (or -1)
+  auto source = [&](int d) -> int {
+    if (d == 13) return -1;
+    const int sx = tx - (d / 9 - 1), sy = ty - ((d / 3) % 3 - 1), sz = tz - (d % 3 - 1);
+    if (sx < 0 || sy < 0 || sz < 0 || sx >= a.T[0] || sy >= a.T[1] || sz >= a.T[2]) return -1;
+    return (sx * a.T[1] + sy) * a.T[2] + sz;
+  };
+  for (int d = w; d < 27; d += kWaves) {
+    int cnt = 0;
+    const int s = source(d);
+    if (s >= 0) {
+      const int m = a.ocnt[s];
+      const int64_t ob = a.ostart[s];
+      for (int c0 = 0; c0 < m; c0 += 64) {
+        const int i = c0 + lane;
+        cnt += __popcll(__ballot(i < m && a.odir[ob + i] == (uint8_t)d));
+      }
+    }
+    if (lane == 0) s_cnt[d] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int o = 0;
+    for (int d = 0; d < 27; ++d) {
+      s_off[d] = o;
+      o += s_cnt[d];
+    }
+    s_off[27] = o;
+  }
+  __syncthreads();
+  const int n0 = a.cnt[t];
   const int cap = a.capa[t];
   const int64_t base = a.start[t];
-  for (int d = 0; d < 27; ++d) {
-    if (d == 13) continue;
-    const int sx = tx - (d / 9 - 1), sy = ty - ((d / 3) % 3 - 1), sz = tz - (d % 3 - 1);
-    if (sx < 0 || sy < 0 || sz < 0 || sx >= a.T[0] || sy >= a.T[1] || sz >= a.T[2]) continue;
-    const int s = (sx * a.T[1] + sy) * a.T[2] + sz;
+  for (int d = w; d < 27; d += kWaves) {
+    const int s = source(d);
+    if (s < 0 || s_cnt[d] == 0) continue;
     const int m = a.ocnt[s];
     const int64_t ob = a.ostart[s];
-    for (int c0 = 0; c0 < m; c0 += kBlock) {
-      const int i = c0 + threadIdx.x;
+    int off = n0 + s_off[d];
+    for (int c0 = 0; c0 < m; c0 += 64) {
+      const int i = c0 + lane;
       const bool hit = i < m && a.odir[ob + i] == (uint8_t)d;
-      int r, r2, tot, tot2;
-      block_rank2(hit, false, r, r2, tot, tot2, sh_rank);
+      const unsigned long long bm = __ballot(hit);
       if (hit) {
         R p[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) p[c] = a.out.c[c][ob + i];
-        const int pos = n + r;
+        const int pos = off + __popcll(bm & lt);
         if (pos < cap) {
 #pragma unroll
           for (int c = 0; c < 6; ++c) a.reg.c[c][base + pos] = p[c];
@@ -730,10 +764,11 @@ __global__ __launch_bounds__(kBlock) void k_arrive(TileArgs<R> a) {
           put_stray(a, p);
         }
       }
-      n = min(n + tot, cap);
+      off += __popcll(bm);
     }
   }
-  if (threadIdx.x == 0) a.cnt[t] = n;
+  __syncthreads();  // every thread has read cnt[t] before it is replaced
+  if (threadIdx.x == 0) a.cnt[t] = min(n0 + s_off[27], cap);
 }
 
 // ------------------------------------------------------------- re-binning
@@ -762,16 +797,21 @@ __global__ void k_keys_tile(SoaPtr<R> f, int64_t n, TileArgs<R> a, uint32_t* key
 
 // Tile starts in a sorted key array: first[u] = index of the first key whose
 // tile (key >> shift, capped at ntiles for invalid particles) is >= u, for
-// u = 0..ntiles. Each thread writes the starts of the tiles that begin at its
-// index, so no two threads write one entry and no atomics are needed.
+// u = 0..ntiles. One thread per tile, by binary search: the cost does not
+// depend on how the keys spread over the tiles (a few staged particles over
+// many empty tiles, or every particle of the species).
 __global__ void k_tile_starts(const uint32_t* keys, int64_t n, int shift, uint32_t ntiles, int64_t* first) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t t = min(keys[i] >> shift, ntiles);
-  const uint32_t tp = i == 0 ? 0u : min(keys[i - 1] >> shift, ntiles) + 1u;
-  for (uint32_t u = tp; u <= t; ++u) first[u] = i;
-  if (i == n - 1)
-    for (uint32_t u = t + 1; u <= ntiles; ++u) first[u] = n;
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u > (int64_t)ntiles) return;
+  int64_t lo = 0, hi = n;  // first index in [lo, hi) whose tile is >= u
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if ((int64_t)min(keys[mid] >> shift, ntiles) < u)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  first[u] = lo;
 }
 
 __global__ void k_i32_from_i64(const int64_t* in, int32_t* out, int n) {
@@ -792,7 +832,7 @@ void count_sorted_tiles(hipStream_t st, const uint32_t* keys, int64_t n, int shi
     FOPIS_HIP(hipMemsetAsync(cnt, 0, (nt + 1) * sizeof(uint32_t), st));
     return;
   }
-  k_tile_starts<<<div_up(n, 256), 256, 0, st>>>(keys, n, shift, (uint32_t)nt, first);
+  k_tile_starts<<<div_up((int64_t)nt + 1, 256), 256, 0, st>>>(keys, n, shift, (uint32_t)nt, first);
   k_counts_from_starts<<<div_up(nt + 1, 256), 256, 0, st>>>(first, (uint32_t)nt, cnt);
 }
 
