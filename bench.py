@@ -220,8 +220,9 @@ def main():
                 % ("device Philox boundary injection" if args.injection == "device" else "host mt19937 injection"),
         "config": {"workload": wl["name"], "particles": int(n_avg), "ppc": args.ppc or None,
                    "poisson_tol": 1e-8, "injection": args.injection,
-                   "mg": "V(2,2) red-black SOR omega=1.5 in %s, FP64 defect correction"
-                         % ("FP32" if h.desc.mg_precision == 1 else "FP64")},
+                   "mg": "V(%d,%d) red-black SOR omega=%g in %s, FP64 defect correction"
+                         % (h.desc.mg_pre, h.desc.mg_post, h.desc.mg_omega,
+                            "FP32" if h.desc.mg_precision == 1 else "FP64")},
         "push_deposit_ms": part_ms,
         "tile_kernel_ms": tile_ms,
         "bin_ms": bin_ms,

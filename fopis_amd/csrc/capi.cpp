@@ -138,7 +138,7 @@ int fopis_create(const fopis_desc* desc, fopis_ctx** out) {
     Ctx& c = h->c;
     c.d = *desc;
     check_desc(c.d);
-    if (c.d.mg_omega == 0.0) c.d.mg_omega = 1.5;
+    if (c.d.mg_omega == 0.0) c.d.mg_omega = 1.25;
     if (c.d.mg_pre == 0) c.d.mg_pre = 2;
     if (c.d.mg_post == 0) c.d.mg_post = 2;
     if (c.d.mg_max_cycles == 0) c.d.mg_max_cycles = 100;

@@ -120,7 +120,7 @@ def device_info() -> str:
 
 
 def make_desc(nx, ny, nz, species, eps0, static_b=(0.0, 0.0, 0.0), boundary="DDDDDD", precision=FP64,
-              mg_precision=None, mg_omega=1.5, mg_pre=2, mg_post=2, mg_max_cycles=100, poisson_tol=1e-8,
+              mg_precision=None, mg_omega=1.25, mg_pre=2, mg_post=2, mg_max_cycles=100, poisson_tol=1e-8,
               capacity_slack=1.25) -> Desc:
     """species: list of (charge, mass, size) in the reference's normalised units."""
     d = Desc()

@@ -74,7 +74,7 @@ typedef struct fopis_desc {
   double eps0;               /* Normalizer::eps0                                        */
   int32_t precision;         /* particles, push and deposit: FOPIS_FP64 | FOPIS_FP32    */
   int32_t mg_precision;      /* V-cycle smoothing: FOPIS_FP64 | FOPIS_FP32              */
-  double mg_omega;           /* red-black SOR relaxation in the V-cycle (0 -> 1.5)      */
+  double mg_omega;           /* red-black SOR relaxation in the V-cycle (0 -> 1.25)     */
   int32_t mg_pre, mg_post;   /* smoothing sweeps (0 -> 2, 2)                            */
   int32_t mg_max_cycles;     /* defect-correction cycle limit (0 -> 100)                */
   double poisson_tol;        /* ||b - A phi||_2 / ||b||_2 target (0 -> 1e-8)            */
