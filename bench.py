@@ -14,6 +14,10 @@ A step is one RootGrid::mainLoopES. `value` is particles/s of the push+deposit
 part of the step (gather, Boris, move, absorb, count, CIC deposit, re-binning
 and the window reduction into rho), timed with HIP events on the library's
 stream inside the timed steps; ms_per_step is the whole step by wall clock.
+The roofline object is k_tile's achieved algorithmic rate against the
+nominal 8 TB/s; hbm_copy_gbs is the device-to-device copy rate measured in
+the same run (the practical streaming ceiling) and roofline.frac_of_copy the
+kernel's rate against it.
 """
 from __future__ import annotations
 
@@ -105,6 +109,32 @@ def learned_leg(h, steps: int, weights: str) -> dict:
     return out
 
 
+def copy_ceiling_gbs(reps: int = 5) -> float:
+    """Device-to-device copy bandwidth (read + write bytes per second) of six
+    512 MiB FP32 arrays, the shape of the particle streams: the HBM rate a
+    plain streaming kernel reaches on this device, for reading the roofline
+    fraction against the nominal 8 TB/s."""
+    import torch
+
+    n = 1 << 27
+    src = [torch.ones(n, dtype=torch.float32, device="cuda") for _ in range(6)]
+    dst = [torch.empty_like(a) for a in src]
+    for a, b in zip(src, dst):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        for a, b in zip(src, dst):
+            b.copy_(a)
+    t1.record()
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * 6 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def pmc_traffic(workload: str):
     p = os.path.join(ROOT, "profiles", f"tile_kernel_{workload}_traffic.json")
     if os.path.exists(p):
@@ -129,9 +159,14 @@ def main():
     ap.add_argument("--guess-weights", default=os.path.join(ROOT, "fopis_amd", "weights", "guess_c4.npz"),
                     help="trained learned-guess weights for the Poisson comparison (skipped if absent)")
     ap.add_argument("--guess-steps", type=int, default=5)
+    ap.add_argument("--copy-ceiling", action="store_true",
+                    help="print only the device-to-device copy rate of six 512 MiB FP32 arrays (GB/s)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (scale for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_ceiling:
+        print(json.dumps({"hbm_copy_gbs": copy_ceiling_gbs()}), flush=True)
+        return
     if args.copy_calibration:
         import torch
 
@@ -250,6 +285,17 @@ def main():
         except Exception as e:  # reported, never silently replaced
             line["poisson_learned_vs_plain"] = {"error": str(e)}
     h.close()
+    # torch ships its own HIP runtime, which cannot share this process with the
+    # library's: the copy runs in a child process
+    try:
+        import subprocess
+
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-ceiling"], capture_output=True,
+                           text=True, timeout=300)
+        line["hbm_copy_gbs"] = float(json.loads(r.stdout.strip().splitlines()[-1])["hbm_copy_gbs"])
+        line["roofline"]["frac_of_copy"] = achieved / line["hbm_copy_gbs"]
+    except Exception as e:  # reported, never silently replaced
+        line["hbm_copy_gbs"] = {"error": str(e)}
     if not args.no_cpu_baseline:
         try:
             threads = min(16, os.cpu_count() or 1)
