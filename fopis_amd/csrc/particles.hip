@@ -46,6 +46,7 @@ struct TileArgs {
   const int64_t* ostart;
   const int32_t* ocapa;
   int32_t* ocnt;
+  int32_t* odcnt;  // [ntiles][27] outbox entries per direction (written by the binning pass)
   const double* in[6];
   const int32_t* ibeg;
   SoaPtr<R> stray;
@@ -371,6 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   using SM = TileSmem<R, MODE>;
   __shared__ __attribute__((aligned(16))) char smem[SM::bytes];
   __shared__ int sh_rank[4 * kWaves];
+  __shared__ int s_dir[(MODE & M_BIN) != 0 ? 27 : 1];  // outbox entries per direction
   R* sE = reinterpret_cast<R*>(smem);
   double* sR = reinterpret_cast<double*>(smem + SM::e_bytes);
   // the count window is its own LDS array so that its atomics stay LDS atomics (ds_add_u32)
@@ -403,6 +405,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   }
   if constexpr (SM::kD) {
     for (int q = threadIdx.x; q < kDW * kDW * kDW; q += kBlock) sD[q] = 0;
+  }
+  if constexpr ((MODE & M_BIN) != 0) {
+    if (threadIdx.x < 27) s_dir[threadIdx.x] = 0;
   }
   __syncthreads();
 
@@ -655,6 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
 #pragma unroll
             for (int c = 0; c < 6; ++c) a.out.c[c][ob] = p[u][c];
             a.odir[ob] = (uint8_t)dir[u];
+            atomicAdd(&s_dir[dir[u]], 1);
           } else {
             put_stray(a, p[u]);
           }
@@ -685,7 +691,10 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
       a.ocnt[t] = optr;
     }
   }
-  if constexpr (SM::kR || SM::kD) __syncthreads();
+  if constexpr (SM::kR || SM::kD || (MODE & M_BIN) != 0) __syncthreads();
+  if constexpr ((MODE & M_BIN) != 0) {
+    if (threadIdx.x < 27) a.odcnt[(size_t)27 * t + threadIdx.x] = s_dir[threadIdx.x];
+  }
   if constexpr (SM::kR) {
     double* P = a.rho_part + (size_t)t * (kRW * kRW * kRW);
     for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) P[q] = sR[q];
@@ -697,10 +706,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
 }
 
 // Arrivals: tile t appends the outbox entries its 26 neighbours addressed to
-// it, neighbour by neighbour in direction order, each in outbox order. Each
-// wavefront owns every kWaves-th direction: it first counts its matches, one
-// scan over the 27 counts gives every direction its offset, then it copies
-// its matches in outbox order (ranks by ballot) — two block barriers in all.
+// it, neighbour by neighbour in direction order, each in outbox order. The
+// binning pass counted every tile's outbox entries per direction, so one scan
+// over the 27 counts gives every direction its offset; each wavefront then
+// copies the matches of every kWaves-th direction in outbox order (ranks by
+// ballot), stopping once it has found them all.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_arrive(TileArgs<R> a) {
   __shared__ int s_cnt[27];
@@ -716,18 +726,9 @@ __global__ __launch_bounds__(kBlock) void k_arrive(TileArgs<R> a) {
     if (sx < 0 || sy < 0 || sz < 0 || sx >= a.T[0] || sy >= a.T[1] || sz >= a.T[2]) return -1;
     return (sx * a.T[1] + sy) * a.T[2] + sz;
   };
-  for (int d = w; d < 27; d += kWaves) {
-    int cnt = 0;
-    const int s = source(d);
-    if (s >= 0) {
-      const int m = a.ocnt[s];
-      const int64_t ob = a.ostart[s];
-      for (int c0 = 0; c0 < m; c0 += 64) {
-        const int i = c0 + lane;
-        cnt += __popcll(__ballot(i < m && a.odir[ob + i] == (uint8_t)d));
-      }
-    }
-    if (lane == 0) s_cnt[d] = cnt;
+  if (threadIdx.x < 27) {
+    const int s = source(threadIdx.x);
+    s_cnt[threadIdx.x] = s < 0 ? 0 : a.odcnt[(size_t)27 * s + threadIdx.x];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -743,12 +744,14 @@ __global__ __launch_bounds__(kBlock) void k_arrive(TileArgs<R> a) {
   const int cap = a.capa[t];
   const int64_t base = a.start[t];
   for (int d = w; d < 27; d += kWaves) {
+    const int want = s_cnt[d];
+    if (want == 0) continue;
     const int s = source(d);
-    if (s < 0 || s_cnt[d] == 0) continue;
     const int m = a.ocnt[s];
     const int64_t ob = a.ostart[s];
     int off = n0 + s_off[d];
-    for (int c0 = 0; c0 < m; c0 += 64) {
+    const int end = off + want;
+    for (int c0 = 0; c0 < m && off < end; c0 += 64) {
       const int i = c0 + lane;
       const bool hit = i < m && a.odir[ob + i] == (uint8_t)d;
       const unsigned long long bm = __ballot(hit);
@@ -1271,6 +1274,7 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.ostart = b.ostart.p;
   a.ocapa = b.ocapa.p;
   a.ocnt = b.ocnt.p;
+  a.odcnt = b.odcnt.p;
   for (int q = 0; q < 6; ++q) a.in[q] = b.inbox.c[q].p;
   a.ibeg = b.inbox_n > 0 ? b.ibeg.p : nullptr;
   a.stray = ptrs(b.stray);
@@ -1328,6 +1332,7 @@ void bin_flat(Ctx& c, int s, Soa<R>& flat, int64_t n, bool key_cell) {
     b.ostart.alloc(nt + 1);
     b.ocapa.alloc(nt + 1);
     b.ocnt.alloc(nt + 1);
+    b.odcnt.alloc((size_t)27 * (nt + 1));
     FOPIS_HIP(hipMemsetAsync(sc.capa.p + nt, 0, sizeof(uint32_t), st));
     FOPIS_HIP(hipMemsetAsync(sc.ocapa.p + nt, 0, sizeof(uint32_t), st));
     exclusive_scan_u32(st, sc.capa.p, nt + 1, b.start.p, sc.tmp);
@@ -1336,6 +1341,7 @@ void bin_flat(Ctx& c, int s, Soa<R>& flat, int64_t n, bool key_cell) {
     k_i32_from_u32<<<div_up(nt + 1, 256), 256, 0, st>>>(sc.ocapa.p, b.ocapa.p, nt + 1);
     k_i32_from_u32<<<div_up(nt + 1, 256), 256, 0, st>>>(sc.cnt.p, b.cnt.p, nt + 1);
     FOPIS_HIP(hipMemsetAsync(b.ocnt.p, 0, (nt + 1) * sizeof(int32_t), st));
+    FOPIS_HIP(hipMemsetAsync(b.odcnt.p, 0, (size_t)27 * (nt + 1) * sizeof(int32_t), st));
     int64_t tot[2];
     FOPIS_HIP(hipMemcpyAsync(&tot[0], b.start.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     FOPIS_HIP(hipMemcpyAsync(&tot[1], b.ostart.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
