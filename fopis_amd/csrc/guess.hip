@@ -629,7 +629,8 @@ __global__ void k_box_charge(double* rho, int cx, int cy, int cz, const uint8_t*
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int c = 0; c < 2; ++c) {
-        const uint8_t m = cm[((size_t)(i - 1 + a) * cy + (j - 1 + b)) * cz + (k - 1 + c)];
+        // the cell map is (cx-1) x (cy-1) x (cz-1): cell (p, q, r) spans nodes p..p+1
+        const uint8_t m = cm[((size_t)(i - 1 + a) * (cy - 1) + (j - 1 + b)) * (cz - 1) + (k - 1 + c)];
         if (m) {
           ++inside;
           owner = m;

@@ -355,13 +355,19 @@ struct TileSmem {
   static constexpr size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
   // E window interleaved per node as (x, y, z, unused): one 16-byte LDS read per corner (FP32)
   static constexpr size_t e_bytes = kE ? align16(sizeof(R) * 4 * kEW * kEW * kEW) : 0;
-  // charge windows accumulate in FP64 for both particle precisions: FP32 LDS
-  // atomics saturate the LDS data FIFO on gfx950 (SQ_WAIT_INST_LDS 50x the
-  // FP64 kernel's), FP64 ones do not
-  static constexpr size_t r_bytes = kR ? align16(sizeof(double) * kRW * kRW * kRW) : 0;
+  // charge windows accumulate 64-bit fixed-point weights for both particle
+  // precisions (ds_add_u64: as fast as FP64 LDS atomics and order-independent;
+  // FP32 LDS atomics are 3x slower on gfx950)
+  static constexpr size_t r_bytes = kR ? align16(sizeof(unsigned long long) * kRW * kRW * kRW) : 0;
   static constexpr size_t bytes = e_bytes + r_bytes + 16;
   static_assert(e_bytes % 16 == 0 && r_bytes % 16 == 0, "LDS segments must stay 16-byte aligned");
 };
+
+// Fixed-point scale of the charge windows: 2^44 units per unit CIC weight. A
+// node of a tile window sums at most 2^19 particles' worth of weight before
+// the 64-bit sum could overflow (the mean is 8 cells x particles per cell);
+// the quantum 2^-44 is 5.7e-14 of a particle's charge.
+constexpr double kFix = 17592186044416.0;
 
 // One workgroup per tile: stage the tile's E nodes in LDS, stream its
 // particles (and the staged new ones) in chunks, apply the requested stages,
@@ -374,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   __shared__ int sh_rank[4 * kWaves];
   __shared__ int s_dir[(MODE & M_BIN) != 0 ? 27 : 1];  // outbox entries per direction
   R* sE = reinterpret_cast<R*>(smem);
-  double* sR = reinterpret_cast<double*>(smem + SM::e_bytes);
+  unsigned long long* sR = reinterpret_cast<unsigned long long*>(smem + SM::e_bytes);
   // the count window is its own LDS array so that its atomics stay LDS atomics (ds_add_u32)
   __shared__ int sD[SM::kD ? kDW * kDW * kDW : 1];
 
@@ -401,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     }
   }
   if constexpr (SM::kR) {
-    for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) sR[q] = 0.0;
+    for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) sR[q] = 0ull;
   }
   if constexpr (SM::kD) {
     for (int q = threadIdx.x; q < kDW * kDW * kDW; q += kBlock) sD[q] = 0;
@@ -571,31 +577,42 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           const int di = min(ci, a.N[0] - 1), dj = min(cj, a.N[1] - 1), dk = min(ck, a.N[2] - 1);
           const R dx1 = p[0] - (R)di, dy1 = p[1] - (R)dj, dz1 = p[2] - (R)dk;
           const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
-          const R q = (R)a.q;
-          const R w[8] = {dx2 * dy2 * dz2 * q, dx1 * dy2 * dz2 * q, dx2 * dy1 * dz2 * q, dx1 * dy1 * dz2 * q,
-                          dx2 * dy2 * dz1 * q, dx1 * dy2 * dz1 * q, dx2 * dy1 * dz1 * q, dx1 * dy1 * dz1 * q};
+          // CIC weights of the eight nodes (the particle's charge is applied when
+          // the window is written out)
+          const R w[8] = {dx2 * dy2 * dz2, dx1 * dy2 * dz2, dx2 * dy1 * dz2, dx1 * dy1 * dz2,
+                          dx2 * dy2 * dz1, dx1 * dy2 * dz1, dx2 * dy1 * dz1, dx1 * dy1 * dz1};
           const int wi = di - o0 + 1, wj = dj - o1 + 1, wk = dk - o2 + 1;
           if (wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW) {
-            double* W = sR + (wi * kRW + wj) * kRW + wk;
-            atomicAdd(W, (double)w[0]);
-            atomicAdd(W + kRW * kRW, (double)w[1]);
-            atomicAdd(W + kRW, (double)w[2]);
-            atomicAdd(W + kRW * kRW + kRW, (double)w[3]);
-            atomicAdd(W + 1, (double)w[4]);
-            atomicAdd(W + kRW * kRW + 1, (double)w[5]);
-            atomicAdd(W + kRW + 1, (double)w[6]);
-            atomicAdd(W + kRW * kRW + kRW + 1, (double)w[7]);
+            // 64-bit fixed point (kFix units per weight 1): integer sums do not
+            // depend on the order the atomics land in, so the window, and rho,
+            // are the same bit for bit from run to run
+            unsigned long long* W = sR + (wi * kRW + wj) * kRW + wk;
+            auto fx = [](R v) -> unsigned long long {
+              if constexpr (sizeof(R) == 4)
+                return __float2ull_rn(v * (float)kFix);  // exact scaling by a power of two
+              else
+                return __double2ull_rn(v * kFix);
+            };
+            atomicAdd(W, fx(w[0]));
+            atomicAdd(W + kRW * kRW, fx(w[1]));
+            atomicAdd(W + kRW, fx(w[2]));
+            atomicAdd(W + kRW * kRW + kRW, fx(w[3]));
+            atomicAdd(W + 1, fx(w[4]));
+            atomicAdd(W + kRW * kRW + 1, fx(w[5]));
+            atomicAdd(W + kRW + 1, fx(w[6]));
+            atomicAdd(W + kRW * kRW + kRW + 1, fx(w[7]));
           } else {
+            const double q = a.q;
             const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
             double* G = a.rho_g + (size_t)(di + 1) * sI + (size_t)(dj + 1) * sJ + (dk + 1);
-            atomicAdd(G, (double)w[0]);
-            atomicAdd(G + sI, (double)w[1]);
-            atomicAdd(G + sJ, (double)w[2]);
-            atomicAdd(G + sI + sJ, (double)w[3]);
-            atomicAdd(G + 1, (double)w[4]);
-            atomicAdd(G + sI + 1, (double)w[5]);
-            atomicAdd(G + sJ + 1, (double)w[6]);
-            atomicAdd(G + sI + sJ + 1, (double)w[7]);
+            atomicAdd(G, q * (double)w[0]);
+            atomicAdd(G + sI, q * (double)w[1]);
+            atomicAdd(G + sJ, q * (double)w[2]);
+            atomicAdd(G + sI + sJ, q * (double)w[3]);
+            atomicAdd(G + 1, q * (double)w[4]);
+            atomicAdd(G + sI + 1, q * (double)w[5]);
+            atomicAdd(G + sJ + 1, q * (double)w[6]);
+            atomicAdd(G + sI + sJ + 1, q * (double)w[7]);
             atomicAdd(&a.status[2], 1);
           }
         }
@@ -697,7 +714,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   }
   if constexpr (SM::kR) {
     double* P = a.rho_part + (size_t)t * (kRW * kRW * kRW);
-    for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) P[q] = sR[q];
+    const double scale = a.q / kFix;  // exact: kFix is a power of two
+    for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) P[q] = (double)(long long)sR[q] * scale;
   }
   if constexpr (SM::kD) {
     int32_t* P = a.dens_part + (size_t)t * (kDW * kDW * kDW);

@@ -130,6 +130,41 @@ def test_network_gradient_matches_oracle(kind):
     ctx.close()
 
 
+def test_synth_boxes_charge_sits_on_box_surfaces():
+    """The synthetic training problems put each box's charge on the nodes
+    whose eight cells are partly inside the box (cells lo <= c < hi of the
+    (n+1)^3 cell map): the support, and the sign of each box's charge, match
+    a numpy count of the inside cells around every interior node."""
+    n = 33
+    ctx = F.Context(_desc(n))
+    g = GU.Guess(ctx, seed=1)
+    shp = ctx.field_shape(F.RHO)
+    ctx.set_field(F.RHO, np.zeros(shp))
+    boxes = [[5, 6, 7, 14, 12, 20], [18, 20, 3, 26, 29, 11]]
+    charges = [0.7, -0.4]
+    g.synth_boxes(boxes, charges, 3)
+    rho = ctx.get_field(F.RHO)
+    cm = np.zeros((n + 1,) * 3, np.int32)
+    for b, bx in enumerate(boxes):
+        cm[bx[0]:bx[3], bx[1]:bx[4], bx[2]:bx[5]] = b + 1
+    cx = n + 2
+    count = np.zeros((cx,) * 3, np.int32)
+    owner = np.zeros((cx,) * 3, np.int32)
+    for a in (0, 1):
+        for b in (0, 1):
+            for c in (0, 1):
+                sl = cm[a:a + cx - 2, b:b + cx - 2, c:c + cx - 2]  # cells (i-1+a, ...) of nodes i = 1..cx-2
+                count[1:cx - 1, 1:cx - 1, 1:cx - 1] += sl > 0
+                owner[1:cx - 1, 1:cx - 1, 1:cx - 1] = np.maximum(owner[1:cx - 1, 1:cx - 1, 1:cx - 1], sl)
+    interior = np.zeros((cx,) * 3, bool)
+    interior[2:cx - 2, 2:cx - 2, 2:cx - 2] = True
+    support = interior & (count > 0) & (count < 8)
+    assert support.sum() > 0
+    np.testing.assert_array_equal(rho != 0, support)
+    assert np.all(rho[support & (owner == 1)] > 0) and np.all(rho[support & (owner == 2)] < 0)
+    ctx.close()
+
+
 def _make_pair(ctx, g, rng, seed, slot=None):
     """rho = plasma-like noise + 1-3 boxes with surface charge; phi from multigrid to 1e-8 from zero.
     Returns the cycles of that solve."""
