@@ -177,10 +177,7 @@ Faces faces(const Ctx& c) {
   return f;
 }
 
-DBuf<double>& sizes_buf(Ctx& c) {
-  static std::map<const Ctx*, DBuf<double>> m;
-  return m[&c];
-}
+DBuf<double>& sizes_buf(Ctx& c) { return c.species_sizes; }
 
 }  // namespace
 

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -219,6 +220,9 @@ struct GuessNet;  // learned initial guess (guess.hip)
 
 struct Ctx {
   fopis_desc d{};
+  // the particle kernels' sort and binning scratch (owned here, so it goes with the context)
+  std::shared_ptr<void> particle_scratch;
+  DBuf<double> species_sizes;  // per-species superparticle sizes for the window reduction
   int S = 0;
   int nx = 0, ny = 0, nz = 0;  // node counts (RootGrid nx)
   int cx = 0, cy = 0, cz = 0;  // with glue

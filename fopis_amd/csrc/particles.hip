@@ -1236,9 +1236,10 @@ struct Scratch {
   DBuf<int64_t> first, tmp;
   DBuf<unsigned long long> ucount;
 };
-std::map<const Ctx*, Scratch> g_scratch;
-
-Scratch& scratch(Ctx& c) { return g_scratch[&c]; }
+Scratch& scratch(Ctx& c) {
+  if (!c.particle_scratch) c.particle_scratch = std::make_shared<Scratch>();
+  return *static_cast<Scratch*>(c.particle_scratch.get());
+}
 
 template <class R>
 SoaPtr<R> ptrs(Soa<R>& s) {
