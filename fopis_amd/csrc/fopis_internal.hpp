@@ -24,6 +24,7 @@ constexpr int kEW = kTile + 1;  // E-field window: nodes of the tile's cells
 constexpr int kRW = kTile + 3;  // charge window: tile cells +-1 after a move
 constexpr int kDW = kTile + 2;  // density window: cells of the tile +-1
 constexpr int kBlock = 256;
+constexpr int kSink = 1 << 16;  // elements of the tile kernel's store sink (a power of two)
 
 struct Error : std::runtime_error {
   int code;
@@ -245,6 +246,7 @@ struct Ctx {
   DBuf<uint8_t> mask;                     // (nx+1)^3 conductor cell ids (0 none, o+1)
   DBuf<uint8_t> tile_near;                // [ntiles] 1 if a conductor cell lies within one tile of the tile
   DBuf<double> cap_rec;                   // captured particles, [S][cap_cap][8]
+  DBuf<double> sink;                      // write target of tile-kernel lanes with no particle to store
   int64_t cap_cap = 0;
   bool have_mask = false;
 

@@ -62,6 +62,7 @@ struct TileArgs {
   ObjArgs obj[FOPIS_MAX_OBJECTS];
   double* cap_rec;  // captured particles: 8 doubles each (state, object)
   int64_t cap_cap;
+  R* sink;  // kSink elements nobody reads (stores of lanes with nothing to keep)
 };
 
 // ------------------------------------------------------------------ helpers
@@ -263,6 +264,20 @@ __global__ void k_capture(const double* rec, int n, int cx, int cy, int cz, doub
 
 constexpr int kWaves = kBlock / 64;  // wavefronts per block
 
+// A uniform pointer as a value held in scalar registers, in the global address
+// space. Selecting among such values keeps a per-lane choice of destination
+// array a register select (a select among the kernel-argument fields
+// themselves can be compiled as a per-lane load of the chosen field), and
+// stores through it are global stores (a generic pointer's would be flat
+// stores, which also count against the LDS wait counter).
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* sgpr_ptr(T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<__attribute__((address_space(1))) T*>(((uint64_t)hi << 32) | lo);
+}
+
 // Ranks of two predicates over the block, in thread order.
 __device__ __forceinline__ void block_rank2(bool pa, bool pb, int& ra, int& rb, int& ta, int& tb, int* sh) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -369,6 +384,17 @@ struct TileSmem {
 // the quantum 2^-44 is 5.7e-14 of a particle's charge.
 constexpr double kFix = 17592186044416.0;
 
+// round(v * kFix) for a CIC weight v in [0, 1] (the value of an exact scaling
+// rounded to the nearest integer, ties to even). v * kFix + 2^52 lies in
+// [2^52, 2^53), where a double's unit in the last place is 1: the one rounding
+// of the fused multiply-add puts the rounded integer in the low 52 bits, and
+// masking off the exponent leaves it. Two instructions instead of a general
+// double-to-64-bit-integer conversion.
+__device__ __forceinline__ unsigned long long fix_weight(double v) {
+  const double m = __builtin_fma(v, kFix, 4503599627370496.0);
+  return (unsigned long long)__double_as_longlong(m) & 0x000FFFFFFFFFFFFFull;
+}
+
 // One workgroup per tile: stage the tile's E nodes in LDS, stream its
 // particles (and the staged new ones) in chunks, apply the requested stages,
 // accumulate charge and counts in LDS windows, and re-bin: particles staying
@@ -430,22 +456,29 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   const bool near_obj = a.nobj > 0 && a.tile_near[t] != 0;
   int wptr = 0, optr = 0;
   const int ocap = ((MODE & M_BIN) != 0) ? a.ocapa[t] : 0;
+  const int64_t ost = ((MODE & M_BIN) != 0) ? a.ostart[t] : 0;
   const int rcap = a.capa[t];
+  const int64_t sink_at = ((int64_t)t * kBlock + threadIdx.x) & (kSink - 1);
 
   // PPT particles per thread and iteration (FP32: two chunks of kBlock, whose
   // loads are in flight together; one block scan ranks both), the next
   // iteration's particles loaded while the current ones are processed
   constexpr int PPT = sizeof(R) == 4 ? 2 : 1;
+  // Every lane loads from the tile region (a lane past the region's particles
+  // reads its first slot, which exists: a region holds at least 64), so each
+  // iteration issues the same loads whatever its lanes hold.
   auto fetch = [&](int idx, R* q) {
-    if (idx < nreg) {
+    const int j = idx < nreg ? idx : 0;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) q[c] = a.reg.c[c][base + idx];
-    } else if (idx < ntot) {
+    for (int c = 0; c < 6; ++c) q[c] = a.reg.c[c][base + j];
+    if (idx >= nreg) {
+      if (idx < ntot) {
 #pragma unroll
-      for (int c = 0; c < 6; ++c) q[c] = (R)a.in[c][ib0 + idx - nreg];
-    } else {
+        for (int c = 0; c < 6; ++c) q[c] = (R)a.in[c][ib0 + idx - nreg];
+      } else {
 #pragma unroll
-      for (int c = 0; c < 6; ++c) q[c] = (R)0;
+        for (int c = 0; c < 6; ++c) q[c] = (R)0;
+      }
     }
   };
   R nxt[PPT][6];
@@ -587,12 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
             // depend on the order the atomics land in, so the window, and rho,
             // are the same bit for bit from run to run
             unsigned long long* W = sR + (wi * kRW + wj) * kRW + wk;
-            auto fx = [](R v) -> unsigned long long {
-              if constexpr (sizeof(R) == 4)
-                return __float2ull_rn(v * (float)kFix);  // exact scaling by a power of two
-              else
-                return __double2ull_rn(v * kFix);
-            };
+            auto fx = [](R v) -> unsigned long long { return fix_weight((double)v); };
             atomicAdd(W, fx(w[0]));
             atomicAdd(W + kRW * kRW, fx(w[1]));
             atomicAdd(W + kRW, fx(w[2]));
@@ -659,31 +687,53 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         block_rank2(stay[0], move[0], rs[0], rm[0], ts[0], tm[0], sh_rank);
       else
         block_rank4(stay[0], move[0], stay[1], move[1], rs, rm, ts, tm, sh_rank);
+      // Every lane issues the same seven stores per particle slot (a lane with
+      // nothing to keep writes to the sink): with a fixed count of memory
+      // operations per iteration, the next particles' loads are waited for
+      // without waiting for this iteration's stores.
       int wb = wptr, ob0 = optr;
 #pragma unroll
       for (int u = 0; u < PPT; ++u) {
+        int kind = 3;  // 0 tile region, 1 outbox, 2 stray list, 3 sink
+        int64_t at = sink_at;
+        bool stray = far[u];
         if (stay[u]) {
           const int pos = wb + rs[u];
           if (pos < rcap) {
-#pragma unroll
-            for (int c = 0; c < 6; ++c) a.reg.c[c][base + pos] = p[u][c];
+            kind = 0;
+            at = base + pos;
           } else {
-            put_stray(a, p[u]);
+            stray = true;
           }
         } else if (move[u]) {
           const int pos = ob0 + rm[u];
           if (pos < ocap) {
-            const int64_t ob = a.ostart[t] + pos;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) a.out.c[c][ob] = p[u][c];
-            a.odir[ob] = (uint8_t)dir[u];
+            kind = 1;
+            at = ost + pos;
             atomicAdd(&s_dir[dir[u]], 1);
           } else {
-            put_stray(a, p[u]);
+            stray = true;
           }
-        } else if (far[u]) {
-          put_stray(a, p[u]);
         }
+        if (stray) {
+          const int slot = atomicAdd(&a.status[0], 1);
+          if (slot < a.stray_cap) {
+            kind = 2;
+            at = slot;
+          } else {
+            atomicAdd(&a.status[1], 1);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          __attribute__((address_space(1))) R* d = kind == 0 ? sgpr_ptr(a.reg.c[c])
+                           : (kind == 1 ? sgpr_ptr(a.out.c[c]) : (kind == 2 ? sgpr_ptr(a.stray.c[c]) : sgpr_ptr(a.sink)));
+          d[at] = p[u][c];
+        }
+        __attribute__((address_space(1))) uint8_t* od =
+            kind == 1 ? sgpr_ptr(a.odir) + at
+                      : reinterpret_cast<__attribute__((address_space(1))) uint8_t*>(sgpr_ptr(a.sink)) + sink_at;
+        *od = (uint8_t)dir[u];
         wb += ts[u];
         ob0 += tm[u];
       }
@@ -1305,6 +1355,7 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.dens_g = c.density.p + (size_t)s * c.ncell_d;
   a.mask = c.mask.p;
   a.tile_near = c.tile_near.p;
+  a.sink = reinterpret_cast<R*>(c.sink.p);
   a.nobj = c.have_mask ? c.nobj : 0;
   for (int o = 0; o < c.nobj; ++o) {
     a.obj[o].node_cmat = c.obj[o].node_cmat.p;
@@ -1561,6 +1612,7 @@ void particles_init(Ctx& c) {
   c.dens_part.alloc((size_t)c.S * c.ntiles * (kDW * kDW * kDW));
   c.cap_cap = 1 << 20;  // captured particles per species and step
   c.cap_rec.alloc((size_t)8 * c.cap_cap * c.S);
+  c.sink.alloc(kSink);  // doubles: kSink elements of either precision, and kSink bytes
   c.status.alloc(4 * FOPIS_MAX_SPECIES);
   FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), c.stream));
   for (int s = 0; s < c.S; ++s) {
