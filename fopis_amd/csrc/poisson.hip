@@ -237,10 +237,16 @@ __global__ void k_smooth(P* __restrict__ e, const P* __restrict__ r, Lv L, int c
     if (ok[q]) e[gg[q]] = val[q];
 }
 
-// First (red) half-sweep of a cycle from e = 0: red points get (w/6) r and
-// black points are cleared, so the level array needs no separate zeroing.
+// First full sweep of a cycle from e = 0, both colours in one pass: red
+// points get (w/6) r, black points the black half-sweep over those red values
+// (formed here from r). The same operations in the same order as a red
+// half-sweep from zero (red points (w/6) r, black points cleared) followed by
+// the black half-sweep, so the same result bit for bit, but r is read once and
+// e written once. Points outside the interior have r = 0 and so read as red
+// values of 0, as the untouched level borders do.
 template <class P>
-__global__ void k_smooth_first(P* __restrict__ e, const P* __restrict__ r, Lv L, P w6, const int* __restrict__ done) {
+__global__ void k_smooth_first2(P* __restrict__ e, const P* __restrict__ r, Lv L, P w1, P w6,
+                                const int* __restrict__ done) {
   if (*done) return;
   int bx, by, bz;
   block3(bx, by, bz);
@@ -252,7 +258,15 @@ __global__ void k_smooth_first(P* __restrict__ e, const P* __restrict__ r, Lv L,
     const int b = by * kRows + q + 1;
     if (b > L.n1) break;
     const int64_t g = a * L.sx + b * L.sy + c;
-    e[g] = ((a + b + c) & 1) == 0 ? w6 * r[g] : (P)0;
+    P v;
+    if (((a + b + c) & 1) == 0) {
+      v = w6 * r[g];
+    } else {
+      const P nb = w6 * r[g - L.sx] + w6 * r[g + L.sx] + w6 * r[g - L.sy] + w6 * r[g + L.sy] + w6 * r[g - 1] +
+                   w6 * r[g + 1];
+      v = w1 * (P)0 + w6 * (nb + r[g]);
+    }
+    e[g] = v;
   }
 }
 
@@ -709,13 +723,14 @@ TailArgs<P> tail_args(Ctx& c, const int* done) {
 template <class P>
 void smooth(Ctx& c, MgLevel& m, int sweeps, double w, bool from_zero, const int* done) {
   const P w1 = (P)(1.0 - w), w6 = (P)(w / 6.0);
-  for (int s = 0; s < sweeps; ++s)
-    for (int col = 0; col < 2; ++col) {
-      if (from_zero && s == 0 && col == 0)
-        k_smooth_first<P><<<grid_full4(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), w6, done);
-      else
-        k_smooth<P><<<grid_half4(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), col, w1, w6, done);
+  for (int s = 0; s < sweeps; ++s) {
+    if (from_zero && s == 0) {
+      k_smooth_first2<P><<<grid_full4(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), w1, w6, done);
+      continue;
     }
+    for (int col = 0; col < 2; ++col)
+      k_smooth<P><<<grid_half4(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), col, w1, w6, done);
+  }
 }
 
 template <class P>
