@@ -303,6 +303,48 @@ __global__ void k_restrict_resid(P* __restrict__ rc, Lv C, const P* __restrict__
   rc[a * C.sx + b * C.sy + c] = acc * (P)0.5;  // (4/8) * sum
 }
 
+// The residual t = r - A e at every interior point of a level, kRows rows per
+// thread (the same expression as the restriction forms on the fly).
+template <class P>
+__global__ void k_resid_level(P* __restrict__ t, const P* __restrict__ e, const P* __restrict__ r, Lv L,
+                              const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int c = bx * blockDim.x + threadIdx.x + 1;
+  const int a = bz + 1;
+  if (c > L.n2) return;
+#pragma unroll
+  for (int q = 0; q < kRows; ++q) {
+    const int b = by * kRows + q + 1;
+    if (b > L.n1) break;
+    const int64_t g = a * L.sx + b * L.sy + c;
+    t[g] = r[g] - ((P)6 * e[g] - (e[g - L.sx] + e[g + L.sx] + e[g - L.sy] + e[g + L.sy] + e[g - 1] + e[g + 1]));
+  }
+}
+
+// Full weighting of a residual already formed (k_resid_level), in the same
+// order as k_restrict_resid: on large levels every fine residual is then
+// computed once instead of once per coarse point that weights it.
+template <class P>
+__global__ void k_restrict_t(P* __restrict__ rc, Lv C, const P* __restrict__ t, Lv F, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int c = bx * blockDim.x + threadIdx.x + 1;
+  const int b = by + 1, a = bz + 1;
+  if (c > C.n2) return;
+  const int64_t f = (2 * a) * F.sx + (2 * b) * F.sy + 2 * c;
+  P acc = 0;
+#pragma unroll
+  for (int da = -1; da <= 1; ++da)
+#pragma unroll
+    for (int db = -1; db <= 1; ++db)
+#pragma unroll
+      for (int dc = -1; dc <= 1; ++dc) acc += fw_weight<P>(da, db, dc) * t[f + da * F.sx + db * F.sy + dc];
+  rc[a * C.sx + b * C.sy + c] = acc * (P)0.5;  // (4/8) * sum
+}
+
 // e_fine += trilinear interpolation of e_coarse (coarse node A sits at fine 2A).
 template <class P>
 __global__ void k_prolong(P* __restrict__ e, Lv F, const P* __restrict__ ec, Lv C, const int* __restrict__ done) {
@@ -656,6 +698,10 @@ dim3 grid_full(const MgLevel& m) { return dim3(div_up(m.n[2], 64), m.n[1], m.n[0
 dim3 grid_full4(const MgLevel& m) { return dim3(div_up(m.n[2], 64), div_up(m.n[1], kRows), m.n[0]); }
 dim3 grid_half4(const MgLevel& m) { return dim3(div_up((m.n[2] + 1) / 2, 64), div_up(m.n[1], kRows), m.n[0]); }
 
+// Levels with at least this many unknowns form their residual in a pass of
+// its own before restricting it (below, one launch forming it on the fly costs less).
+constexpr int64_t kSplitRestrict = (int64_t)1 << 20;
+
 int coarse_sweeps(const MgLevel& m) { return 4 * std::max({m.n[0], m.n[1], m.n[2]}) + 8; }
 
 // First level handled by the LDS tail kernel and its LDS bytes, for element size `esz`.
@@ -758,7 +804,13 @@ void vcycle(Ctx& c, int l, const int* done) {
   // every level starts its cycle from e = 0 (the correction equation)
   if (c.d.mg_pre == 0) k_zero_level<P><<<grid_full(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), done);
   smooth<P>(c, m, c.d.mg_pre, w, true, done);
-  k_restrict_resid<P><<<grid_full(mc), 64, 0, c.stream>>>(R_<P>(mc), lv_of(mc), E_<P>(m), R_<P>(m), lv_of(m), done);
+  if ((int64_t)m.n[0] * m.n[1] * m.n[2] >= kSplitRestrict) {
+    k_resid_level<P><<<grid_full4(m), 64, 0, c.stream>>>(T_<P>(m), E_<P>(m), R_<P>(m), lv_of(m), done);
+    k_restrict_t<P><<<grid_full(mc), 64, 0, c.stream>>>(R_<P>(mc), lv_of(mc), T_<P>(m), lv_of(m), done);
+  } else {
+    k_restrict_resid<P><<<grid_full(mc), 64, 0, c.stream>>>(R_<P>(mc), lv_of(mc), E_<P>(m), R_<P>(m), lv_of(m),
+                                                            done);
+  }
   vcycle<P>(c, l + 1, done);
   k_prolong<P><<<grid_full4(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), E_<P>(mc), lv_of(mc), done);
   smooth<P>(c, m, c.d.mg_post, w, false, done);
