@@ -51,6 +51,10 @@ hid_t native(int type) {
       return H5T_NATIVE_UINT64;
     case FOPIS_IO_I32:
       return H5T_NATIVE_INT32;
+    case FOPIS_IO_F32:
+      return H5T_NATIVE_FLOAT;
+    case FOPIS_IO_U8:
+      return H5T_NATIVE_UINT8;
     default:
       throw H5Error("unknown element type");
   }

@@ -148,6 +148,42 @@ class Guess:
         _check(self.L, self.L.fopis_guess_dataset_set(self.ctx.h, slot, _p(r, C.c_uint16), _p(m, C.c_uint8),
                                                       _p(p, C.c_uint16)))
 
+    def export_pairs(self, path: str, begin: int, end: int) -> int:
+        """Writes dataset slots [begin, end) to an HDF5 file in the layout the
+        reference's own files use (groups and plain datasets, as in its resume
+        files): grid (int32, 3) and count (int32 scalar), then per pair i
+        pairs/<i:06d>/rho_hat and phi_hat (float32, the network grid; the
+        bf16 values widened exactly) and mask (uint8)."""
+        from fopis_amd.h5 import H5File
+
+        with H5File(path, "w") as f:
+            f.write("grid", np.array(self.grid, np.int32))
+            f.write("count", np.int32(end - begin))
+            for i in range(begin, end):
+                rho, mask, phi = self.dataset_get(i)
+                k = f"pairs/{i - begin:06d}"
+                f.write(k + "/rho_hat", from_bf16(rho).reshape(self.grid))
+                f.write(k + "/mask", mask)
+                f.write(k + "/phi_hat", from_bf16(phi).reshape(self.grid))
+        return end - begin
+
+    def import_pairs(self, path: str, slot0: int = 0) -> int:
+        """Reads a file written by export_pairs into slots slot0, slot0 + 1, ...
+        (bit for bit: the stored values are bf16 values). The network grid must
+        match; the dataset must have room."""
+        from fopis_amd.h5 import H5File
+
+        with H5File(path, "r") as f:
+            grid = tuple(int(v) for v in f.read("grid", np.int32))
+            if grid != tuple(self.grid):
+                raise ValueError(f"pairs for grid {grid}, network grid {tuple(self.grid)}")
+            n = int(f.read("count", np.int32))
+            for i in range(n):
+                k = f"pairs/{i:06d}"
+                self.dataset_set(slot0 + i, to_bf16(f.read(k + "/rho_hat", np.float32)), f.read(k + "/mask", np.uint8),
+                                 to_bf16(f.read(k + "/phi_hat", np.float32)))
+        return n
+
     def grad(self, slot: int):
         g = np.empty(self.nparams, np.float32)
         loss = C.c_double()

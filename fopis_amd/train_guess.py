@@ -10,7 +10,9 @@ Training, inference and the pair store all run in libfopis_hip.so.
 
 Writes <out>.npz (params float32, out_scale, grid, and the recipe) and prints
 one JSON line with the losses and, on held-out problems, multigrid cycles and
-milliseconds from zero and from the network's guess.
+milliseconds from zero and from the network's guess. --pairs-out writes the
+generated (rho_hat, mask, phi_hat) pairs to an HDF5 file and --pairs-in trains
+on such a file instead of generating (Guess.export_pairs / import_pairs).
 
   python -m fopis_amd.train_guess --config fopis_amd/configs/c2.json \
       --ntrain 2000 --nheld 200 --epochs 8 --out fopis_amd/weights/guess_c4
@@ -114,6 +116,9 @@ def main(argv=None):
                     help="0: potential MSE; 1: the multigrid's starting residual (sets its cycle count)")
     ap.add_argument("--init", default="", help="start from these weights (.npz)")
     ap.add_argument("--out", default="")
+    ap.add_argument("--pairs-out", default="", help="also write the generated pairs to this HDF5 file")
+    ap.add_argument("--pairs-in", default="",
+                    help="train on the pairs of this HDF5 file (written by --pairs-out) instead of generating them")
     args = ap.parse_args(argv)
 
     cfg = plasma_config(args.config)
@@ -126,24 +131,39 @@ def main(argv=None):
     if args.init:
         z = np.load(args.init, allow_pickle=False)
         g.set_params(z["params"], float(z["out_scale"]))
-    ntot = args.ntrain + args.nheld
-    g.dataset_alloc(ntot)
     pm = PairMaker(h, g, args.seed, args.box_min, args.box_max, args.plasma_every)
     log(f"grid {g.grid}, {g.nparams} parameters, rho rms {pm.rho_rms:.4g}, setup {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     cycles = []
-    for i in range(ntot):
-        pm.setup(i)
-        cyc, rr, _ = pm.solve()
-        if not rr <= 1e-8:
-            raise RuntimeError(f"pair {i}: solve stopped at relres {rr}")
-        cycles.append(cyc)
-        g.dataset_store(i)
-        if (i + 1) % max(1, ntot // 10) == 0:
-            log(f"pairs {i + 1}/{ntot} ({time.perf_counter() - t0:.1f} s)")
+    if args.pairs_in:
+        from fopis_amd.h5 import H5File
+
+        with H5File(args.pairs_in, "r") as f:
+            nfile = int(f.read("count", np.int32))
+        if nfile < args.ntrain + args.nheld:
+            raise ValueError(f"{args.pairs_in} holds {nfile} pairs, {args.ntrain} + {args.nheld} asked for")
+        ntot = nfile
+        g.dataset_alloc(ntot)
+        g.import_pairs(args.pairs_in)
+        log(f"read {ntot} pairs from {args.pairs_in}")
+    else:
+        ntot = args.ntrain + args.nheld
+        g.dataset_alloc(ntot)
+        for i in range(ntot):
+            pm.setup(i)
+            cyc, rr, _ = pm.solve()
+            if not rr <= 1e-8:
+                raise RuntimeError(f"pair {i}: solve stopped at relres {rr}")
+            cycles.append(cyc)
+            g.dataset_store(i)
+            if (i + 1) % max(1, ntot // 10) == 0:
+                log(f"pairs {i + 1}/{ntot} ({time.perf_counter() - t0:.1f} s)")
+        if args.pairs_out:
+            g.export_pairs(args.pairs_out, 0, ntot)
+            log(f"wrote {ntot} pairs to {args.pairs_out}")
     gen_s = time.perf_counter() - t0
 
-    held = (args.ntrain, ntot) if args.nheld else (0, args.ntrain)
+    held = (args.ntrain, args.ntrain + args.nheld) if args.nheld else (0, args.ntrain)
     held_before = g.eval(*held)
     t0 = time.perf_counter()
     losses = []
@@ -173,7 +193,7 @@ def main(argv=None):
     res = {
         "grid": list(g.grid), "params": int(g.nparams), "out_scale": k,
         "pairs": {"train": args.ntrain, "held_out": args.nheld, "generate_s": gen_s,
-                  "cycles_from_zero_mean": float(np.mean(cycles))},
+                  "cycles_from_zero_mean": float(np.mean(cycles)) if cycles else None},
         "loss": {"train_first_epoch": losses[0] if losses else None, "train_last_epoch": losses[-1] if losses else None,
                  "held_out_before": held_before, "held_out_after": held_after},
         "train_s": train_s, "steps_per_s": args.epochs * args.ntrain / max(train_s, 1e-9),

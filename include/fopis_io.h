@@ -28,8 +28,9 @@ extern "C" {
 typedef struct fopis_h5 fopis_h5;
 
 /* Element types, as the reference's HighFive calls store them:
- * F64 = createDataSet<double>, U64 = createDataSet<size_t>, I32 = createDataSet<int>. */
-enum fopis_io_type { FOPIS_IO_F64 = 0, FOPIS_IO_U64 = 1, FOPIS_IO_I32 = 2 };
+ * F64 = createDataSet<double>, U64 = createDataSet<size_t>, I32 = createDataSet<int>;
+ * F32 and U8 (float, uint8_t) hold the learned guess's training pairs. */
+enum fopis_io_type { FOPIS_IO_F64 = 0, FOPIS_IO_U64 = 1, FOPIS_IO_I32 = 2, FOPIS_IO_F32 = 3, FOPIS_IO_U8 = 4 };
 
 /* mode 0: open an existing file read-only; 1: create or truncate (H5F::Create | H5F::Truncate). */
 int fopis_io_open(const char* path, int mode, fopis_h5** out);

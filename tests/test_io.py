@@ -22,7 +22,7 @@ H5DUMP = shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/
 
 pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="libfopis_io.so not built (no HDF5)")
 
-F64, U64, I32 = 0, 1, 2
+F64, U64, I32, F32, U8 = 0, 1, 2, 3, 4
 
 
 class IO:
@@ -80,6 +80,8 @@ def test_round_trip_types_and_ranks(tmp_path):
         "capacity_matrix": (rng.random((17, 17)), F64, np.float64),
         "Potential/phi": (rng.random((5, 6, 7)), F64, np.float64),
         "Bfield/bx": (np.zeros((0, 0, 0)), F64, np.float64),
+        "pairs/000000/rho_hat": (rng.standard_normal((4, 5, 6)).astype(np.float32), F32, np.float32),
+        "pairs/000000/mask": (rng.integers(0, 3, (4, 5, 6)).astype(np.uint8), U8, np.uint8),
     }
     for name, (a, t, _) in data.items():
         io.write(h, name, a, t)
@@ -124,3 +126,27 @@ def test_h5dump_sees_the_reference_layout(tmp_path):
     assert re.search(r'DATASET "capacity_matrix".*?DATASPACE\s+SIMPLE \{ \( 4, 4 \)', out, re.S)
     assert re.search(r'DATASET "grid_spacing".*?DATASPACE\s+SCALAR', out, re.S)
     assert 'GROUP "electron"' in out
+
+
+def test_package_binding_round_trip(tmp_path):
+    """fopis_amd.h5 (the package's binding) picks the element type from the
+    array's dtype and reads back bit for bit; a missing dataset raises."""
+    from fopis_amd.h5 import H5File
+
+    rng = np.random.default_rng(8)
+    data = {"grid": np.array([8, 16, 24], np.int32), "count": np.int32(2),
+            "pairs/000001/phi_hat": rng.standard_normal((8, 16, 24)).astype(np.float32),
+            "pairs/000001/mask": (rng.random((8, 16, 24)) < 0.1).astype(np.uint8),
+            "capacity_matrix": rng.random((5, 5)), "generated": np.arange(6, dtype=np.uint64)}
+    p = tmp_path / "b.h5"
+    with H5File(p, "w") as f:
+        for k, v in data.items():
+            f.write(k, v)
+    with H5File(p, "r") as f:
+        for k, v in data.items():
+            got = f.read(k, np.asarray(v).dtype)
+            assert got.shape == np.shape(v)
+            np.testing.assert_array_equal(got, v)
+        assert f.exists("pairs/000001/mask") and not f.exists("pairs/000002/mask")
+        with pytest.raises(RuntimeError):
+            f.read("pairs/000002/mask", np.uint8)
