@@ -3,7 +3,7 @@
 // CPU restatement of hsimyu/fopis's electrostatic PIC step (RootGrid::mainLoopES
 // and everything it calls), written from the reference's algorithm, not copied.
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
-// it, and only as the checker or the reported CPU baseline: the product
+// it, and only to check results against or as the reported CPU baseline: the product
 // (fopis_amd/libfopis_hip.so) never links, loads or calls anything here.
 //
 // Every function names the reference file:line it restates. Arithmetic order
