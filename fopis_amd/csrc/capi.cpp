@@ -117,7 +117,8 @@ int fopis_device_info(char* buf, int buflen) {
     FOPIS_HIP(hipGetDevice(&dev));
     hipDeviceProp_t pr;
     FOPIS_HIP(hipGetDeviceProperties(&pr, dev));
-    std::string s = std::string("libfopis_hip 0.1; device ") + pr.name + " (" + pr.gcnArchName + "), " +
+    const std::string name = pr.name[0] ? std::string(pr.name) + " " : std::string();
+    std::string s = std::string("libfopis_hip 0.1; device ") + name + "(" + pr.gcnArchName + "), " +
                     std::to_string(pr.multiProcessorCount) + " CUs, " +
                     std::to_string((long long)(pr.totalGlobalMem >> 20)) + " MiB";
     if (buf && buflen > 0) {
