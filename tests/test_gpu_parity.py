@@ -255,6 +255,31 @@ def test_multi_step_against_oracle_fp64():
     assert abs(fe_gpu - g.L.rc_efield_energy(g.h)) <= 1e-5 * abs(fe_gpu)
 
 
+def test_forty_steps_track_the_oracle_fp64():
+    """Forty whole steps (no conductor, no injection): the same particles
+    leave through the faces on both sides, and the particle state, the
+    kinetic and the field energy stay with the oracle's. The two Poisson
+    solvers agree to ~1e-10, and the difference grows to ~1e-8 in 40 steps;
+    the bars leave a factor of 100."""
+    g = _oracle(ppc=4)
+    ctx = _ctx(g)
+    _upload(ctx, g)
+    g.call("rc_initialize")
+    ctx.initialize()
+    for _ in range(40):
+        g.call("rc_main_loop")
+        ctx.main_loop_es()
+    for s in range(g.S):
+        a, b = canon(ctx.get_particles(s)), canon(g.get_particles(s))
+        assert a.shape == b.shape
+        assert np.max(np.abs(a - b)) < 1e-6
+    assert ctx.num_particles(0) < (g.nx - 1) ** 3 * 4 * 0.9  # over a tenth of the electrons left
+    ke, fe = ctx.energy()
+    assert abs(ke - g.L.rc_particle_energy(g.h)) <= 1e-8 * abs(ke)
+    assert abs(fe - g.L.rc_efield_energy(g.h)) <= 1e-6 * abs(fe)
+    ctx.close()
+
+
 def test_far_movers_and_crowded_tile_match_oracle():
     """Particles that jump more than a tile in one step go through the stray
     list and deposit outside their tile's charge window; a crowd converging
