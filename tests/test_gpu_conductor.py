@@ -205,3 +205,36 @@ def test_lshape_capture_and_cmatrix(tmp_path):
     for s in range(g.S):
         np.testing.assert_array_equal(canon(ctx.get_particles(s)), canon(g.get_particles(s)))
     ctx.close()
+
+
+def test_thirty_steps_with_conductor_and_injection_track_the_oracle(tmp_path):
+    """Thirty whole steps through the host facade (host mt19937 injection, a
+    floating conductor with the oracle's C): the conductor's charge, its
+    potential, the particle count and both energies stay with the oracle's.
+    Measured: the charge equal to the last digit, potential and energies to
+    1e-7 and better; the bars leave a margin of 10 to 100."""
+    from fopis_amd.host import Host
+
+    cfg = _cfg(17)
+    p = tmp_path / "c2l.json"
+    p.write_text(json.dumps(cfg))
+    g = _oracle(ppc=6)
+    h = Host(str(p), nx_override=17, ppc_override=6, build_cmatrix=True, initialize=False)
+    ctx = h.ctx
+    ctx.set_cmatrix(g.cmatrix(0), 0)
+    ctx.initialize()
+    g.call("rc_initialize")
+    for _ in range(30):
+        h.main_loop_es()
+        g.call("rc_main_loop")
+    st, info = ctx.object_state(0), g.object_info(0)
+    q_dev, q_ref = st["charge_map"].sum(), g.object_charge(0).sum()
+    assert abs(q_dev - q_ref) <= 1e-9 * abs(q_ref)
+    assert abs(st["potential"] - info["potential"]) <= 1e-5 * abs(info["potential"])
+    ke, fe = ctx.energy()
+    assert abs(ke - g.L.rc_particle_energy(g.h)) <= 1e-8 * abs(ke)
+    assert abs(fe - g.L.rc_efield_energy(g.h)) <= 1e-6 * abs(fe)
+    for s in range(g.S):
+        a, b = canon(ctx.get_particles(s)), canon(g.get_particles(s))  # the oracle's invalid particles dropped
+        assert abs(len(a) - len(b)) <= 1e-3 * len(b)
+    h.close()
