@@ -149,9 +149,9 @@ class Guess:
                                                       _p(p, C.c_uint16)))
 
     def export_pairs(self, path: str, begin: int, end: int) -> int:
-        """Writes dataset slots [begin, end) to an HDF5 file in the layout the
-        reference's own files use (groups and plain datasets, as in its resume
-        files): grid (int32, 3) and count (int32 scalar), then per pair i
+        """Writes dataset slots [begin, end) to an HDF5 file laid out as the
+        reference's resume files are (groups of plain datasets,
+        RootGridIO.cpp:25-290): grid (int32, 3) and count (int32 scalar), then per pair i
         pairs/<i:06d>/rho_hat and phi_hat (float32, the network grid; the
         bf16 values widened exactly) and mask (uint8)."""
         from fopis_amd.h5 import H5File

@@ -1,6 +1,7 @@
 """HDF5 datasets through libfopis_io.so (include/fopis_io.h): plain
 datasets of rank 0 to 3 in C order, paths with intermediate groups
-("Potential/phi"), in the element types the reference's files use."""
+("Potential/phi"), in the element types the reference's files use
+(src/Misc/IO.cpp:28-120, RootGridIO.cpp:25-290)."""
 from __future__ import annotations
 
 import ctypes as C
