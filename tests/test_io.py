@@ -117,12 +117,14 @@ def test_h5dump_sees_the_reference_layout(tmp_path):
     io.write(h, "time_spacing", np.float64(1e-7), F64)
     io.write(h, "electron/generated_counts", np.zeros(6, np.uint64), U64)
     io.write(h, "timestep", np.int32(3), I32)
+    io.write(h, "pairs/000000/rho_hat", np.zeros((2, 2, 2), np.float32), F32)
+    io.write(h, "pairs/000000/mask", np.zeros((2, 2, 2), np.uint8), U8)
     io.L.fopis_io_close(h)
     out = subprocess.run([H5DUMP, "-H", str(p)], capture_output=True, text=True, check=True).stdout
     blocks = dict(re.findall(r'DATASET "([^"]+)" \{\s*DATATYPE\s+(\S+)', out))
     assert blocks == {"capacity_matrix": "H5T_IEEE_F64LE", "grid_spacing": "H5T_IEEE_F64LE",
                       "time_spacing": "H5T_IEEE_F64LE", "generated_counts": "H5T_STD_U64LE",
-                      "timestep": "H5T_STD_I32LE"}
+                      "timestep": "H5T_STD_I32LE", "rho_hat": "H5T_IEEE_F32LE", "mask": "H5T_STD_U8LE"}
     assert re.search(r'DATASET "capacity_matrix".*?DATASPACE\s+SIMPLE \{ \( 4, 4 \)', out, re.S)
     assert re.search(r'DATASET "grid_spacing".*?DATASPACE\s+SCALAR', out, re.S)
     assert 'GROUP "electron"' in out
