@@ -1,14 +1,16 @@
 """Benchmark of the MI355X ES PIC step (one GPU).
 
-Default workload: 128^3 cells, 64 particles per cell for each of electrons
-and ions (2 x 133M superparticles, FP64), a floating 16^3-cell cube conductor
-handled by the capacitance matrix, upstream x-face injection, multigrid
-Poisson (FP32 V(2,2) cycles, FP64 defect correction) to relative residual
-1e-8 (two solves per step because of the conductor). Particles are drawn on
-the device (Philox Maxwellian, synthetic); boundary injection keeps the
-reference's per-face counts and residues on the host and draws the particles
-on the device (--injection host draws them on the host from the reference's
-mt19937 streams instead). --workload c3 / c5 run the 256^3 FP32 cases.
+Default workload (C3): 256^3 cells, 64 particles per cell for each of
+electrons and ions (2 x 1.07e9 superparticles, FP32 positions and
+velocities), a floating 32^3-cell cube conductor handled by the capacitance
+matrix, B0 = 0.2 along z (full Boris rotation), upstream x-face injection,
+multigrid Poisson (FP32 V(2,2) cycles, FP64 defect correction) to relative
+residual 1e-8 (two solves per step because of the conductor). Particles are
+drawn on the device (Philox Maxwellian, synthetic); boundary injection keeps
+the reference's per-face counts and residues on the host and draws the
+particles on the device (--injection host draws them on the host from the
+reference's mt19937 streams instead). --workload c2 runs the 128^3 FP64
+case, c5 the 256^3 L-shaped spacecraft with photoelectrons.
 
 A step is one RootGrid::mainLoopES. `value` is particles/s of the push+deposit
 part of the step (gather, Boris, move, absorb, count, CIC deposit, re-binning
@@ -17,7 +19,10 @@ stream inside the timed steps; ms_per_step is the whole step by wall clock.
 The roofline object is k_tile's achieved algorithmic rate against the
 nominal 8 TB/s; hbm_copy_gbs is the device-to-device copy rate measured in
 the same run (the practical streaming ceiling) and roofline.frac_of_copy the
-kernel's rate against it.
+kernel's rate against it. cpu_baseline times the oracle's push + move +
+deposit on every core the process may use; poisson_cpu_baseline times the
+reference's SOR on the right-hand side of the GPU's first solve of a step,
+warm-started from the previous step's phi.
 """
 from __future__ import annotations
 
@@ -50,30 +55,102 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg_name: str, threads: int, ppc: int = 4, steps: int = 2) -> dict:
-    """The oracle (CPU restatement of the reference step, AoS FP64, OpenMP over
-    particles) timed on push + move + deposit at the workload's grid with a
-    reduced particle count."""
+def host_info() -> dict:
+    """The host the CPU baseline runs on: logical CPUs, those this process
+    may run on, the cgroup CPU quota, the CPU model and the RAM."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+        info["cgroup_cpus"] = None if q == "max" else float(q) / float(per)
+    except (OSError, ValueError):
+        info["cgroup_cpus"] = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            info["model"] = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        info["model"] = None
+    try:
+        with open("/proc/meminfo") as f:
+            kb = next(int(l.split()[1]) for l in f if l.startswith("MemTotal"))
+        info["ram_gb"] = round(kb / 2 ** 20, 1)
+    except (OSError, StopIteration):
+        info["ram_gb"] = None
+    env = os.environ.get("OMP_NUM_THREADS")
+    info["omp_num_threads_env"] = int(env) if env and env.isdigit() else None
+    return info
+
+
+def cpu_threads(info: dict) -> int:
+    """Every core this process may use: the affinity set, bounded by the
+    cgroup quota and by OMP_NUM_THREADS when the box sets it (its CPU share)."""
+    n = info["affinity"] or 1
+    if info["cgroup_cpus"]:
+        n = min(n, max(1, int(info["cgroup_cpus"])))
+    if info["omp_num_threads_env"]:
+        n = min(n, info["omp_num_threads_env"])
+    return n
+
+
+def cpu_baseline(cfg_name: str, threads: int, info: dict, ppc: int = 2, samples: int = 5) -> dict:
+    """The oracle (CPU restatement of the reference step: AoS FP64 particles,
+    OpenMP over particles for the push and move, over species for the
+    deposit) timed on push + move + deposit at the workload's grid with a
+    reduced particle count; median of `samples` steps."""
     from oracle import refcpu
     from tests.cases import load_config, oracle_params
 
     refcpu.set_threads(threads)
     cfg = load_config(cfg_name)
     g = refcpu.Grid(oracle_params(cfg, ppc=ppc))
+    t0 = time.perf_counter()
     g.call("rc_load_initial_particles")
+    load_s = time.perf_counter() - t0
     n = sum(g.num_particles(s) for s in range(g.S))
     ts = []
-    for _ in range(steps):
+    for _ in range(samples):
         t0 = time.perf_counter()
         g.call("rc_update_velocity")
         g.call("rc_update_position")
         g.call("rc_deposit_only")
         ts.append(time.perf_counter() - t0)
-    best = min(ts)
+    med = sorted(ts)[len(ts) // 2]
     nx = cfg["Environment"]["nx"]
-    return {"value": n / best, "unit": "particles/s", "cores": threads, "kind": "port",
-            "sample": f"oracle push+move+deposit (reference algorithm, FP64 AoS) on the {nx - 1}^3 grid with the "
-                      f"conductor, {ppc} ppc per species ({n} particles), best of {steps} steps of {best:.2f} s"}
+    del g
+    return {"value": n / med, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": f"oracle push+move+deposit (reference algorithm and layout: FP64 AoS, OpenMP over particles, "
+                      f"deposit parallel over species) on the {nx - 1}^3 grid with the conductor, {ppc} ppc per "
+                      f"species ({n} particles), median of {samples} steps",
+            "step_s": ts, "load_s": load_s, "host": info}
+
+
+def poisson_cpu_baseline(desc, cfg_name: str, phi_prev, rho_new, threads: int, gpu_ms: float,
+                         gpu_cycles: float) -> dict:
+    """The reference's Poisson solver (zebra SOR, omega = 2/(1 + sin(pi/n)),
+    RootGridField.cpp:65-264) in the oracle on the same right-hand side the
+    GPU's first solve of a step had, warm-started from the previous step's
+    phi, as the reference warm-starts: (i) with the reference's own stop (the
+    pointwise relative residual below 1e-7 checked every 10 sweeps, at most
+    1000 sweeps), (ii) to this build's ||b - A phi||_2 / ||b||_2 <= 1e-8."""
+    from oracle import refcpu
+    from tests.cases import load_config, oracle_params
+
+    refcpu.set_threads(threads)
+    g = refcpu.Grid(oracle_params(load_config(cfg_name), ppc=1, with_objects=False))
+    out = {"unit": "ms", "cores": threads, "kind": "port", "gpu_first_solve_ms": gpu_ms,
+           "gpu_first_solve_cycles": gpu_cycles}
+    g.set_field(refcpu.RHO, rho_new, 0)
+    for key, run in (("reference_stop", lambda: g.call("rc_solve_poisson")),
+                     ("relres_1e-8", lambda: g.call("rc_solve_poisson_relres", 1e-8, 100000))):
+        g.set_field(refcpu.PHI, phi_prev)
+        t0 = time.perf_counter()
+        sweeps = run()
+        dt = time.perf_counter() - t0
+        out[key] = {"ms": dt * 1e3, "sweeps": int(sweeps), "relres": float(g.L.rc_relres(g.h))}
+    out["value"] = out["relres_1e-8"]["ms"]
+    out["gpu_speedup_to_1e-8"] = out["relres_1e-8"]["ms"] / gpu_ms if gpu_ms > 0 else None
+    del g
+    return out
 
 
 def learned_leg(h, steps: int, weights: str) -> dict:
@@ -89,8 +166,10 @@ def learned_leg(h, steps: int, weights: str) -> dict:
     out = {"weights": os.path.relpath(weights, ROOT)}
     z = np.load(weights, allow_pickle=False)
     g = Guess(ctx, seed=1)
-    if tuple(int(v) for v in z["grid"]) != tuple(g.grid):
-        return {"skipped": f"weights are for grid {tuple(z['grid'])}, this grid is {g.grid}"}
+    # the convolution weights do not depend on the grid size: weights trained
+    # on one grid run on any other
+    out["trained_grid"] = [int(v) for v in z["grid"]]
+    out["grid"] = [int(v) for v in g.grid]
     g.set_params(z["params"], float(z["out_scale"]))
     for mode, key in ((0, "plain"), (1, "learned")):
         g.set_mode(mode)
@@ -163,7 +242,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--ppc", type=int, default=0, help="override particles per cell (0 = config)")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="",
                     help="override the workload's particle precision (profiling the other kernel instance)")
@@ -296,6 +375,15 @@ def main():
         # HBM bytes of one k_tile launch (one species) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
         line["roofline"]["traffic"] = tr.get("bytes_per_launch")
         line["roofline"]["algorithmic_bytes_per_launch"] = bpp * n_avg / h.S
+    # the first solve of one more step, for the CPU Poisson baseline: its
+    # right-hand side (the deposited charge, before the conductor correction)
+    # and its warm start (the previous step's phi)
+    poisson_case = None
+    if not args.no_cpu_baseline:
+        phi_prev = ctx.get_field(F.PHI)
+        h.main_loop_es()
+        rho_new = sum(ctx.get_field(F.RHO, s + 1) for s in range(h.S))
+        poisson_case = (phi_prev, rho_new)
     if args.guess_steps > 0 and os.path.exists(args.guess_weights):
         try:
             line["poisson_learned_vs_plain"] = learned_leg(h, args.guess_steps, args.guess_weights)
@@ -314,11 +402,20 @@ def main():
     except Exception as e:  # reported, never silently replaced
         line["hbm_copy_gbs"] = {"error": str(e)}
     if not args.no_cpu_baseline:
+        info = host_info()
+        threads = cpu_threads(info)
+        log(f"cpu baseline on {threads} threads ({info})")
         try:
-            threads = min(16, os.cpu_count() or 1)
-            line["cpu_baseline"] = cpu_baseline(args.workload, threads)
-        except Exception as e:  # the oracle is optional on a box without it
+            line["cpu_baseline"] = cpu_baseline(args.workload, threads, info)
+        except Exception as e:  # reported, never silently replaced
             line["cpu_baseline"] = {"error": str(e)}
+        log("cpu Poisson baseline")
+        try:
+            line["poisson_cpu_baseline"] = poisson_cpu_baseline(
+                h.desc, args.workload, poisson_case[0], poisson_case[1], threads, first_ms,
+                stats["first_cycles"] / max(1, args.steps))
+        except Exception as e:  # reported, never silently replaced
+            line["poisson_cpu_baseline"] = {"error": str(e)}
     print(json.dumps(line), flush=True)
 
 

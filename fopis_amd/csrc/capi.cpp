@@ -75,6 +75,9 @@ static void update_rho(Ctx& c, bool push) {
     if (guess_mode(c) == 1) guess_apply(c);
     poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
     c.mg_first_cycles_total += c.mg_cycles_last;
+    // the object charge is redistributed from this potential: it must have converged
+    if (!(rr <= c.d.poisson_tol))
+      throw Error(FOPIS_E_NOCONV, "first Poisson solve stopped at relres " + std::to_string(rr));
     objects_redistribute(c);
   }
 }
@@ -623,6 +626,15 @@ int fopis_nn_conv3d_backward(int32_t n0, int32_t n1, int32_t n2, int32_t cin, in
     return FOPIS_E_ARG;
   }
   return guarded([&] { nn_conv3d_backward(n0, n1, n2, cin, cout, up, x, w, gy, dw, db, dx); });
+}
+
+int fopis_particle_ledger(fopis_ctx* ctx, int s, int64_t* out) {
+  NEED_CTX
+  return guarded([&] {
+    if (s < 0 || s >= ctx->c.S) throw Error(FOPIS_E_ARG, "species out of range");
+    if (!out) throw Error(FOPIS_E_ARG, "null output");
+    particles_ledger(ctx->c, s, out);
+  });
 }
 
 int fopis_synchronize(fopis_ctx* ctx) {

@@ -25,6 +25,11 @@ constexpr int kRW = kTile + 3;  // charge window: tile cells +-1 after a move
 constexpr int kDW = kTile + 2;  // density window: cells of the tile +-1
 constexpr int kBlock = 256;
 constexpr int kSink = 1 << 16;  // elements of the tile kernel's store sink (a power of two)
+// Per-species status words of the particle kernels: [0] strays, [1] lost
+// (stray or capture buffer full), [2] deposits outside the tile window,
+// [3] captured by a conductor, [4] absorbed at a domain face, [5] staged
+// particles that joined the tile regions.
+constexpr int kStat = 8;
 
 struct Error : std::runtime_error {
   int code;
@@ -174,6 +179,10 @@ struct SpeciesBins {
   Soa<R> stray;          // particles that did not fit a bin this step
   int64_t stray_cap = 0;
   int64_t total_cap = 0;
+  bool has_invalid = false;  // a standalone position update left absorbed particles (x = NaN) in their slots
+  // particle ledger since the context was made: staged particles that joined,
+  // particles absorbed at a domain face, particles captured by a conductor
+  int64_t joined = 0, absorbed = 0, captured = 0;
 };
 
 struct ObjectDev {
@@ -296,6 +305,7 @@ void particles_sort_by_cell(Ctx& c);
 void particles_cell_keys(Ctx& c, int s, int64_t cap, int32_t* keys, int64_t* n);
 double particles_energy(Ctx& c);
 void particles_moments(Ctx& c, int s, double* out);
+void particles_ledger(Ctx& c, int s, int64_t* out);
 
 void fields_init(Ctx& c);
 void fields_efield(Ctx& c);

@@ -8,6 +8,8 @@
 //   perfect-conductor charge   src/Spacecraft/Spacecraft.cpp:708-758
 //   Cmatrix::total             include/Spacecraft/Cmatrix.hpp:36-44
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "fopis_internal.hpp"
 
@@ -339,24 +341,200 @@ static void finish_cmatrix(Ctx& c, ObjectDev& ob) {
   FOPIS_HIP(hipStreamSynchronize(c.stream));
 }
 
+// ---- the unit-charge responses B from the exact discrete Green's function
+//
+// With Dirichlet faces on every axis, the 7-point operator A (6 phi minus the
+// six neighbours, dx = 1) on the m0 x m1 x m2 interior nodes is diagonalised
+// by the sine transform: A = S diag(lambda) S with S = S0 (x) S1 (x) S2,
+// S_d[k][u] = sqrt(2/(m_d+1)) sin(pi k u/(m_d+1)) (k, u = 1..m_d, orthogonal
+// and symmetric) and lambda(k) = sum_d 4 sin^2(pi k_d / (2(m_d+1))). The
+// response at node i to a unit charge at node j (RootGridObject.cpp:102-125
+// solves A phi = rho/eps0 with rho = 1 at j) is therefore
+//   B(j,i) = (1/eps0) sum_k prod_d S_d[k_d][u_d(i)] S_d[k_d][u_d(j)] / lambda(k),
+// the limit the reference's unit solves approach. The surface nodes lie in a
+// box of w_d nodes per axis, and the sum separates over the axes:
+//   T[k_b][a_c,b_c][k_a] = sum_{k_c} S_c[k_c][a_c] S_c[k_c][b_c] / lambda   (GEMM, batched over k_b)
+//   U[a_b,b_b][a_c,b_c][k_a] = sum_{k_b} S_b[k_b][a_b] S_b[k_b][b_b] T       (one GEMM)
+//   B(j,i) = (1/eps0) sum_{k_a} S_a[k_a][a_a(i)] S_a[k_a][a_a(j)] U          (one wavefront per pair)
+// with axis c the one of smallest box width and a the one of largest. FP64
+// throughout; the GEMMs are plain LDS-tiled FMA kernels.
+
+// C[b] (M x N) = A (M x K) x B[b] (K x N), row-major, A shared by the batch.
+constexpr int kGT = 64, kGK = 16;
+__global__ __launch_bounds__(256) void k_dgemm(const double* __restrict__ A, const double* __restrict__ Bm,
+                                               double* __restrict__ Cm, int M, int N, int K, int64_t sB,
+                                               int64_t sC) {
+  __shared__ double As[kGK][kGT + 1];
+  __shared__ double Bs[kGK][kGT + 1];
+  const double* Bb = Bm + (int64_t)blockIdx.z * sB;
+  double* Cb = Cm + (int64_t)blockIdx.z * sC;
+  const int m0 = blockIdx.y * kGT, n0 = blockIdx.x * kGT;
+  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;  // 4x4 outputs: rows tr+16u, cols tc+16v
+  double acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += kGK) {
+    for (int q = threadIdx.x; q < kGK * kGT; q += 256) {
+      const int kk = q / kGT, mm = q % kGT;
+      // A tile transposed into As[k][m] (reads along k are strided: A is small)
+      As[kk][mm] = (m0 + mm < M && k0 + kk < K) ? A[(int64_t)(m0 + mm) * K + k0 + kk] : 0.0;
+      Bs[kk][mm] = (k0 + kk < K && n0 + mm < N) ? Bb[(int64_t)(k0 + kk) * N + n0 + mm] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kGK; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = As[kk][tr + 16 * u];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) b[v] = Bs[kk][tc + 16 * v];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = __builtin_fma(a[u], b[v], acc[u][v]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = m0 + tr + 16 * u, col = n0 + tc + 16 * v;
+      if (r < M && col < N) Cb[(int64_t)r * N + col] = acc[u][v];
+    }
+}
+
+// invl[kb][kc][ka] = 1 / (la[ka] + lb[kb] + lc[kc])
+__global__ void k_inv_lambda(const double* la, const double* lb, const double* lc, int ma, int mb, int mc,
+                             double* invl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)ma * mb * mc) return;
+  const int ka = (int)(i % ma), kc = (int)((i / ma) % mc), kb = (int)(i / ((int64_t)ma * mc));
+  invl[i] = 1.0 / (la[ka] + lb[kb] + lc[kc]);
+}
+
+// P[(a,b)][k] = S[k][a] S[k][b] for a, b < w
+__global__ void k_pair_products(const double* S, int m, int w, double* P) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)w * w * m) return;
+  const int k = (int)(i % m), ab = (int)(i / m), a = ab / w, b = ab % w;
+  P[i] = S[(int64_t)k * w + a] * S[(int64_t)k * w + b];
+}
+
+// B(col,row): one wavefront per (col, row) pair, lanes over k_a
+__global__ __launch_bounds__(256) void k_green_pairs(const double* __restrict__ U, const double* __restrict__ Sa,
+                                                     const int32_t* __restrict__ box, int n, int ma, int wa,
+                                                     int wb, int wc, double scale, double* __restrict__ B) {
+  const int64_t pair = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (pair >= (int64_t)n * n) return;
+  const int col = (int)(pair / n), row = (int)(pair % n);
+  const int aa = box[3 * col], ab = box[3 * col + 1], ac = box[3 * col + 2];
+  const int ba = box[3 * row], bb = box[3 * row + 1], bc = box[3 * row + 2];
+  const double* u = U + (((int64_t)ab * wb + bb) * (wc * wc) + (int64_t)ac * wc + bc) * ma;
+  double s = 0.0;
+  for (int k = lane; k < ma; k += 64) s += Sa[(int64_t)k * wa + aa] * Sa[(int64_t)k * wa + ba] * u[k];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) B[(size_t)col * n + row] = s * scale;
+}
+
+// Fills B (n x n, (col,row) at col*n+row) from the sine-transform Green's
+// function; false when a face is not Dirichlet or a node is not an interior
+// unknown (the caller then solves for the columns).
+static bool spectral_unit_responses(Ctx& c, ObjectDev& ob, double* B) {
+  for (int q = 0; q < 6; ++q)
+    if (c.d.boundary[q] != 'D') return false;
+  const int n = ob.ncmat;
+  const int m[3] = {c.nx - 2, c.ny - 2, c.nz - 2};
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+  for (int q = 0; q < n; ++q)
+    for (int d = 0; d < 3; ++d) {
+      const int u = ob.cmat_ijk[3 * q + d] - 1;  // interior unknown index along d (1..m)
+      if (u < 1 || u > m[d]) return false;
+      lo[d] = std::min(lo[d], u);
+      hi[d] = std::max(hi[d], u);
+    }
+  int w[3], ord[3] = {0, 1, 2};
+  for (int d = 0; d < 3; ++d) w[d] = hi[d] - lo[d] + 1;
+  // ord = {a, b, c}: a largest width, c smallest
+  std::sort(ord, ord + 3, [&](int x, int y) { return w[x] > w[y] || (w[x] == w[y] && x < y); });
+  const int A_ = ord[0], B_ = ord[1], C_ = ord[2];
+  hipStream_t st = c.stream;
+  // sine tables S_d[k][a] (node lo_d + a) and eigenvalues, in FP64 on the host
+  DBuf<double> S[3], L[3];
+  for (int d = 0; d < 3; ++d) {
+    const double md = (double)(m[d] + 1);
+    std::vector<double> hs((size_t)m[d] * w[d]), hl(m[d]);
+    const double nrm = std::sqrt(2.0 / md);
+    for (int k = 1; k <= m[d]; ++k) {
+      const double h = std::sin(M_PI * k / (2.0 * md));
+      hl[k - 1] = 4.0 * h * h;
+      for (int a = 0; a < w[d]; ++a) hs[(size_t)(k - 1) * w[d] + a] = nrm * std::sin(M_PI * k * (double)(lo[d] + a) / md);
+    }
+    S[d].alloc(hs.size());
+    L[d].alloc(hl.size());
+    FOPIS_HIP(hipMemcpyAsync(S[d].p, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    FOPIS_HIP(hipMemcpyAsync(L[d].p, hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  const int ma = m[A_], mb = m[B_], mc = m[C_], wa = w[A_], wb = w[B_], wc = w[C_];
+  DBuf<double> invl, Pc, Pb, T, U;
+  invl.alloc((size_t)mb * mc * ma);
+  k_inv_lambda<<<div_up((int64_t)ma * mb * mc, 256), 256, 0, st>>>(L[A_].p, L[B_].p, L[C_].p, ma, mb, mc, invl.p);
+  Pc.alloc((size_t)wc * wc * mc);
+  k_pair_products<<<div_up((int64_t)wc * wc * mc, 256), 256, 0, st>>>(S[C_].p, mc, wc, Pc.p);
+  Pb.alloc((size_t)wb * wb * mb);
+  k_pair_products<<<div_up((int64_t)wb * wb * mb, 256), 256, 0, st>>>(S[B_].p, mb, wb, Pb.p);
+  // T[kb] (wc^2 x ma) = Pc (wc^2 x mc) x invl[kb] (mc x ma)
+  T.alloc((size_t)mb * wc * wc * ma);
+  k_dgemm<<<dim3(div_up(ma, kGT), div_up(wc * wc, kGT), mb), 256, 0, st>>>(
+      Pc.p, invl.p, T.p, wc * wc, ma, mc, (int64_t)mc * ma, (int64_t)wc * wc * ma);
+  invl.release();
+  // U (wb^2 x wc^2 ma) = Pb (wb^2 x mb) x T (mb x wc^2 ma)
+  const int64_t nU = (int64_t)wc * wc * ma;
+  if (nU > INT32_MAX) return false;
+  U.alloc((size_t)wb * wb * nU);
+  k_dgemm<<<dim3(div_up(nU, kGT), div_up(wb * wb, kGT), 1), 256, 0, st>>>(Pb.p, T.p, U.p, wb * wb, (int)nU, mb, 0,
+                                                                          0);
+  T.release();
+  // each node's box coordinates in (a, b, c) order
+  std::vector<int32_t> hbox(3 * (size_t)n);
+  for (int q = 0; q < n; ++q) {
+    hbox[3 * q] = ob.cmat_ijk[3 * q + A_] - 1 - lo[A_];
+    hbox[3 * q + 1] = ob.cmat_ijk[3 * q + B_] - 1 - lo[B_];
+    hbox[3 * q + 2] = ob.cmat_ijk[3 * q + C_] - 1 - lo[C_];
+  }
+  DBuf<int32_t> box;
+  box.alloc(hbox.size());
+  FOPIS_HIP(hipMemcpyAsync(box.p, hbox.data(), hbox.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  const int64_t pairs = (int64_t)n * n;
+  k_green_pairs<<<div_up(pairs, 4), 256, 0, st>>>(U.p, S[A_].p, box.p, n, ma, wa, wb, wc, 1.0 / c.d.eps0, B);
+  FOPIS_HIP(hipGetLastError());
+  FOPIS_HIP(hipStreamSynchronize(st));
+  return true;
+}
+
 void objects_build_cmatrix(Ctx& c, int o) {
   ObjectDev& ob = c.obj[o];
   const int n = ob.ncmat;
   if (n <= 0) return;
   hipStream_t st = c.stream;
   const int64_t nn = (int64_t)c.nnode;
-  std::vector<int64_t> nodes(n);
-  FOPIS_HIP(hipMemcpy(nodes.data(), ob.cmat_node.p, n * sizeof(int64_t), hipMemcpyDeviceToHost));
   DBuf<double> B;
   B.alloc((size_t)n * n);
-  k_zero_d<<<div_up(nn, 256), 256, 0, st>>>(c.phi.p, nn);
-  for (int col = 0; col < n; ++col) {
-    k_unit_rhs<<<div_up(nn, 256), 256, 0, st>>>(c.rho.p, nn, nodes[col]);
-    double rr = 0.0;
-    poisson_solve(c, 1e-11, 400, &rr, c.rho.p);
-    k_gather_col<<<div_up(n, 256), 256, 0, st>>>(B.p, n, col, c.phi.p, ob.cmat_node.p);
+  if (!spectral_unit_responses(c, ob, B.p)) {
+    // one unit-charge multigrid solve per surface node (RootGridObject.cpp:102-125)
+    std::vector<int64_t> nodes(n);
+    FOPIS_HIP(hipMemcpy(nodes.data(), ob.cmat_node.p, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+    k_zero_d<<<div_up(nn, 256), 256, 0, st>>>(c.phi.p, nn);
+    for (int col = 0; col < n; ++col) {
+      k_unit_rhs<<<div_up(nn, 256), 256, 0, st>>>(c.rho.p, nn, nodes[col]);
+      double rr = 0.0;
+      poisson_solve(c, 1e-11, 400, &rr, c.rho.p);
+      k_gather_col<<<div_up(n, 256), 256, 0, st>>>(B.p, n, col, c.phi.p, ob.cmat_node.p);
+    }
+    k_zero_d<<<div_up(nn, 256), 256, 0, st>>>(c.rho.p, nn);
+  } else {
+    // the reference leaves phi and rho cleared after the unit solves (RootGridObject.cpp:83-85)
+    k_zero_d<<<div_up(nn, 256), 256, 0, st>>>(c.phi.p, nn);
   }
-  k_zero_d<<<div_up(nn, 256), 256, 0, st>>>(c.rho.p, nn);
   // C = B^-1 by Gauss-Jordan with partial pivoting
   k_identity<<<div_up((int64_t)n * n, 256), 256, 0, st>>>(ob.C.p, n);
   DBuf<double> f;

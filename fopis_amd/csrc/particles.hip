@@ -14,6 +14,8 @@
 // FP64 arithmetic is written in the reference's operation order and compiled
 // without contraction, so an FP64 push reproduces the reference bit for bit
 // on identical inputs.
+#include <cmath>
+
 #include "fopis_internal.hpp"
 #include "radix.hpp"
 
@@ -51,7 +53,7 @@ struct TileArgs {
   const int32_t* ibeg;
   SoaPtr<R> stray;
   int64_t stray_cap;
-  int32_t* status;  // [0] strays, [1] lost, [2] out-of-window deposits
+  int32_t* status;  // kStat words of this species (fopis_internal.hpp)
   double* rho_part;
   int32_t* dens_part;
   double* rho_g;
@@ -380,9 +382,12 @@ struct TileSmem {
 
 // Fixed-point scale of the charge windows: 2^44 units per unit CIC weight. A
 // node of a tile window sums at most 2^19 particles' worth of weight before
-// the 64-bit sum could overflow (the mean is 8 cells x particles per cell);
-// the quantum 2^-44 is 5.7e-14 of a particle's charge.
+// the signed 64-bit sum could overflow (the mean is 8 cells x particles per
+// cell); a tile that processes kFixMaxParticles or more particles in one
+// launch deposits through the global FP64 path instead. The quantum 2^-44 is
+// 5.7e-14 of a particle's charge.
 constexpr double kFix = 17592186044416.0;
+constexpr int kFixMaxParticles = 1 << 19;
 
 // round(v * kFix) for a CIC weight v in [0, 1] (the value of an exact scaling
 // rounded to the nearest integer, ties to even). v * kFix + 2^52 lies in
@@ -453,6 +458,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     }
   }
   const int ntot = nreg + nin;
+  if constexpr ((MODE & M_BIN) != 0) {
+    if (threadIdx.x == 0 && nin > 0) atomicAdd(&a.status[5], nin);  // staged particles joining the regions
+  }
+  // every weight this tile's window sums comes from these ntot particles
+  const bool fix_ok = ntot < kFixMaxParticles;
   const bool near_obj = a.nobj > 0 && a.tile_near[t] != 0;
   int wptr = 0, optr = 0;
   const int ocap = ((MODE & M_BIN) != 0) ? a.ocapa[t] : 0;
@@ -487,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
 
   // one particle through velocity, position, cell, density, capture and
   // deposit; returns whether it is still valid (its cell in ci, cj, ck)
-  auto work = [&](R (&p)[6], int idx, int& ci, int& cj, int& ck) -> bool {
+  auto work = [&](R (&p)[6], int idx, int& ci, int& cj, int& ck, bool& lost) -> bool {
     const bool act = idx < ntot;
     const bool from_in = idx >= nreg;
     bool valid = act && !isnan(p[0]);
@@ -552,8 +562,10 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           p[1] = p[1] + p[4];
           p[2] = p[2] + p[5];
           if (p[0] < (R)0 || p[0] > (R)a.xmax[0] || p[1] < (R)0 || p[1] > (R)a.xmax[1] || p[2] < (R)0 ||
-              p[2] > (R)a.xmax[2])
+              p[2] > (R)a.xmax[2]) {
             valid = false;
+            lost = true;
+          }
         }
       }
 
@@ -615,7 +627,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           const R w[8] = {dx2 * dy2 * dz2, dx1 * dy2 * dz2, dx2 * dy1 * dz2, dx1 * dy1 * dz2,
                           dx2 * dy2 * dz1, dx1 * dy2 * dz1, dx2 * dy1 * dz1, dx1 * dy1 * dz1};
           const int wi = di - o0 + 1, wj = dj - o1 + 1, wk = dk - o2 + 1;
-          if (wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW) {
+          if (fix_ok && wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW) {
             // 64-bit fixed point (kFix units per weight 1): integer sums do not
             // depend on the order the atomics land in, so the window, and rho,
             // are the same bit for bit from run to run
@@ -657,10 +669,20 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
       for (int c = 0; c < 6; ++c) p[u][c] = nxt[u][c];
 #pragma unroll
     for (int u = 0; u < PPT; ++u) fetch(c0 + (PPT + u) * kBlock + threadIdx.x, nxt[u]);
-    bool valid[PPT];
+    bool valid[PPT], lost[PPT];
     int ci[PPT], cj[PPT], ck[PPT];
 #pragma unroll
-    for (int u = 0; u < PPT; ++u) valid[u] = work(p[u], c0 + u * kBlock + threadIdx.x, ci[u], cj[u], ck[u]);
+    for (int u = 0; u < PPT; ++u) {
+      lost[u] = false;
+      valid[u] = work(p[u], c0 + u * kBlock + threadIdx.x, ci[u], cj[u], ck[u], lost[u]);
+    }
+    if constexpr ((MODE & M_POS) != 0) {
+      // particles absorbed at a domain face, one atomic per wavefront
+      int nl = 0;
+#pragma unroll
+      for (int u = 0; u < PPT; ++u) nl += __popcll(__ballot(lost[u]));
+      if ((threadIdx.x & 63) == 0 && nl > 0) atomicAdd(&a.status[4], nl);
+    }
 
     if constexpr ((MODE & M_BIN) != 0) {
       int dir[PPT];
@@ -993,6 +1015,9 @@ __global__ __launch_bounds__(kBlock) void k_v2(SoaPtr<R> f, int64_t n, double* p
   __shared__ double s[kBlock];
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    // a particle that left the domain in a standalone position update keeps its
+    // slot with x = NaN until the next re-bin; Grid::getParticleEnergy skips it
+    if (isnan(f.c[0][i])) continue;
     const double vx = f.c[3][i], vy = f.c[4][i], vz = f.c[5][i];
     acc += vx * vx + vy * vy + vz * vz;
   }
@@ -1032,6 +1057,18 @@ __global__ __launch_bounds__(kBlock) void k_moments(SoaPtr<R> f, const int64_t* 
     __syncthreads();
   }
   if (threadIdx.x < 9) part[(size_t)blockIdx.x * 9 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Live particles of each tile region (x not NaN), one workgroup per tile.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_count_valid(const R* x, const int64_t* start, const int32_t* cnt,
+                                                        unsigned long long* out) {
+  const int64_t b0 = start[blockIdx.x];
+  const int n = cnt[blockIdx.x];
+  int k = 0;
+  for (int i = threadIdx.x; i < n; i += kBlock) k += isnan(x[b0 + i]) ? 0 : 1;
+  for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o);
+  if ((threadIdx.x & 63) == 0 && k) atomicAdd(out, (unsigned long long)k);
 }
 
 __global__ void k_sum_serial(const double* part, int n, double* out) {
@@ -1348,7 +1385,7 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.ibeg = b.inbox_n > 0 ? b.ibeg.p : nullptr;
   a.stray = ptrs(b.stray);
   a.stray_cap = b.stray_cap;
-  a.status = c.status.p + 4 * s;
+  a.status = c.status.p + kStat * s;
   a.rho_part = c.rho_part.p + (size_t)s * c.ntiles * (kRW * kRW * kRW);
   a.dens_part = c.dens_part.p + (size_t)s * c.ntiles * (kDW * kDW * kDW);
   a.rho_g = c.rho.p + (size_t)(s + 1) * c.nnode;
@@ -1428,6 +1465,7 @@ void bin_flat(Ctx& c, int s, Soa<R>& flat, int64_t n, bool key_cell) {
       b.stray_cap = want_stray;
     }
   }
+  if (!key_cell) b.has_invalid = false;
   if (n > 0) {
     SoaPtr<R> dst = ptrs(b.reg);
     k_fill_regions<R><<<div_up(n, 256), 256, 0, st>>>(dst, b.start.p, ptrs(flat), sc.keys.p, sc.perm.p, n, shift,
@@ -1453,7 +1491,7 @@ int64_t gather_all(Ctx& c, int s, Soa<R>& flat, bool with_strays) {
   int64_t nreg = 0;
   int32_t stat[4] = {0, 0, 0, 0};
   FOPIS_HIP(hipMemcpyAsync(&nreg, sc.first.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  FOPIS_HIP(hipMemcpyAsync(stat, c.status.p + 4 * s, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  FOPIS_HIP(hipMemcpyAsync(stat, c.status.p + kStat * s, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   FOPIS_HIP(hipStreamSynchronize(st));
   if (stat[1] > 0) throw Error(FOPIS_E_NOMEM, "stray buffer overflow: particles were lost");
   const int64_t nstray = with_strays ? std::min<int64_t>(stat[0], b.stray_cap) : 0;
@@ -1469,7 +1507,7 @@ template <class R>
 void rebin(Ctx& c, int s) {
   Soa<R> flat;
   const int64_t n = gather_all<R>(c, s, flat, true);
-  FOPIS_HIP(hipMemsetAsync(c.status.p + 4 * s, 0, 4 * sizeof(int32_t), c.stream));
+  FOPIS_HIP(hipMemsetAsync(c.status.p + kStat * s, 0, kStat * sizeof(int32_t), c.stream));
   bin_flat<R>(c, s, flat, n, false);
   FOPIS_HIP(hipStreamSynchronize(c.stream));
 }
@@ -1539,15 +1577,20 @@ void launch_tiles(Ctx& c, int s) {
 
 template <class R>
 void after_bin(Ctx& c) {
-  std::vector<int32_t> stat(4 * c.S);
-  FOPIS_HIP(hipMemcpyAsync(stat.data(), c.status.p, 4 * c.S * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  std::vector<int32_t> stat(kStat * c.S);
+  FOPIS_HIP(hipMemcpyAsync(stat.data(), c.status.p, kStat * c.S * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
   FOPIS_HIP(hipStreamSynchronize(c.stream));
   for (auto& b : bins<R>(c)) b.upload_pending = false;
-  for (int s = 0; s < c.S; ++s)
-    if (stat[4 * s + 1] > 0) throw Error(FOPIS_E_NOMEM, "stray or capture buffer overflow: particles were lost");
+  for (int s = 0; s < c.S; ++s) {
+    if (stat[kStat * s + 1] > 0) throw Error(FOPIS_E_NOMEM, "stray or capture buffer overflow: particles were lost");
+    auto& b = bins<R>(c)[s];
+    b.captured += stat[kStat * s + 3];
+    b.absorbed += stat[kStat * s + 4];
+    b.joined += stat[kStat * s + 5];
+  }
   // surface charge of the captured particles (Spacecraft::distributeInnerParticleCharge)
   for (int s = 0; s < c.S; ++s) {
-    const int n = stat[4 * s + 3];
+    const int n = stat[kStat * s + 3];
     if (n <= 0 || c.nobj == 0) continue;
     ObjArgs oa[FOPIS_MAX_OBJECTS] = {};
     for (int o = 0; o < c.nobj; ++o) {
@@ -1562,7 +1605,7 @@ void after_bin(Ctx& c) {
   }
   // strays -> full re-bin of that species
   for (int s = 0; s < c.S; ++s)
-    if (stat[4 * s + 0] > 0) rebin<R>(c, s);
+    if (stat[kStat * s + 0] > 0) rebin<R>(c, s);
 }
 
 template <class R>
@@ -1592,7 +1635,10 @@ void deposit_bin(Ctx& c, bool push) {
     }
     FOPIS_HIP(hipGetLastError());
   }
-  for (int s = 0; s < c.S; ++s) bins<R>(c)[s].inbox_n = 0;
+  for (int s = 0; s < c.S; ++s) {
+    bins<R>(c)[s].inbox_n = 0;
+    bins<R>(c)[s].has_invalid = false;  // re-binning dropped every invalid particle
+  }
   after_bin<R>(c);
   {
     Phase ph(c, "reduce");
@@ -1613,7 +1659,7 @@ void particles_init(Ctx& c) {
   c.cap_cap = 1 << 20;  // captured particles per species and step
   c.cap_rec.alloc((size_t)8 * c.cap_cap * c.S);
   c.sink.alloc(kSink);  // doubles: kSink elements of either precision, and kSink bytes
-  c.status.alloc(4 * FOPIS_MAX_SPECIES);
+  c.status.alloc(kStat * FOPIS_MAX_SPECIES);
   FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), c.stream));
   for (int s = 0; s < c.S; ++s) {
     if (c.fp32)
@@ -1644,7 +1690,7 @@ static void set_impl(Ctx& c, int s, int64_t n, const double* const* h) {
     k_from_f64<R><<<div_up(n, 256), 256, 0, st>>>(ptrs(flat), ptrs(raw), n);
     FOPIS_HIP(hipStreamSynchronize(st));
   }
-  FOPIS_HIP(hipMemsetAsync(c.status.p + 4 * s, 0, 4 * sizeof(int32_t), st));
+  FOPIS_HIP(hipMemsetAsync(c.status.p + kStat * s, 0, kStat * sizeof(int32_t), st));
   bin_flat<R>(c, s, flat, n, false);
   FOPIS_HIP(hipStreamSynchronize(st));
 }
@@ -1652,6 +1698,13 @@ static void set_impl(Ctx& c, int s, int64_t n, const double* const* h) {
 template <class R>
 static void load_impl(Ctx& c, int s, int64_t n, uint64_t seed, const double* lo, const double* hi,
                       const double* drift, double sigma) {
+  // the species is replaced as a whole, as in set_impl: particles staged for
+  // the next deposit go too (after any queued upload from the host staging)
+  auto& b = bins<R>(c)[s];
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  b.host_inbox.n = 0;
+  b.upload_pending = false;
+  b.dev_n = 0;
   Soa<R> flat;
   if (n > 0) {
     flat.alloc(n);
@@ -1660,7 +1713,7 @@ static void load_impl(Ctx& c, int s, int64_t n, uint64_t seed, const double* lo,
         make_double3(drift[0], drift[1], drift[2]), sigma, c.have_mask ? c.mask.p : nullptr, c.cy - 1, c.cz - 1);
     FOPIS_HIP(hipGetLastError());
   }
-  FOPIS_HIP(hipMemsetAsync(c.status.p + 4 * s, 0, 4 * sizeof(int32_t), c.stream));
+  FOPIS_HIP(hipMemsetAsync(c.status.p + kStat * s, 0, kStat * sizeof(int32_t), c.stream));
   bin_flat<R>(c, s, flat, n, false);
   FOPIS_HIP(hipStreamSynchronize(c.stream));
 }
@@ -1703,11 +1756,25 @@ static int64_t staged_inside(Ctx& c, SpeciesBins<R>& b) {
 template <class R>
 static int64_t count_impl(Ctx& c, int s) {
   auto& b = bins<R>(c)[s];
-  std::vector<int32_t> cnt(c.ntiles);
-  FOPIS_HIP(hipMemcpyAsync(cnt.data(), b.cnt.p, c.ntiles * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
-  FOPIS_HIP(hipStreamSynchronize(c.stream));
   int64_t n = 0;
-  for (int v : cnt) n += v;
+  if (b.has_invalid) {
+    // Grid::getValidParticleNumber counts valid particles only: a standalone
+    // position update left the absorbed ones in their slots (x = NaN)
+    Scratch& sc = scratch(c);
+    sc.ucount.alloc(1);
+    FOPIS_HIP(hipMemsetAsync(sc.ucount.p, 0, sizeof(unsigned long long), c.stream));
+    k_count_valid<R><<<c.ntiles, kBlock, 0, c.stream>>>(b.reg.c[0].p, b.start.p, b.cnt.p, sc.ucount.p);
+    FOPIS_HIP(hipGetLastError());
+    unsigned long long k = 0;
+    FOPIS_HIP(hipMemcpyAsync(&k, sc.ucount.p, sizeof(k), hipMemcpyDeviceToHost, c.stream));
+    FOPIS_HIP(hipStreamSynchronize(c.stream));
+    n = (int64_t)k;
+  } else {
+    std::vector<int32_t> cnt(c.ntiles);
+    FOPIS_HIP(hipMemcpyAsync(cnt.data(), b.cnt.p, c.ntiles * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    FOPIS_HIP(hipStreamSynchronize(c.stream));
+    for (int v : cnt) n += v;
+  }
   return n + (int64_t)b.host_inbox.n + staged_inside(c, b);
 }
 
@@ -1744,14 +1811,35 @@ static void get_impl(Ctx& c, int s, int64_t cap, double* const* h, int64_t* nout
     }
     if (ok) keep.push_back(i);
   }
-  *nout = n + nh + (int64_t)keep.size();
+  // resident particles; a standalone position update may have left absorbed
+  // ones (x = NaN) in their slots until the next re-bin: they are not copied
+  int64_t nv = n;
+  std::vector<double> tmp;
+  if (b.has_invalid && n > 0) {
+    tmp.resize(6 * (size_t)n);
+    for (int q = 0; q < 6; ++q)
+      FOPIS_HIP(hipMemcpyAsync(tmp.data() + q * (size_t)n, out.c[q].p, n * sizeof(double), hipMemcpyDeviceToHost,
+                               c.stream));
+    FOPIS_HIP(hipStreamSynchronize(c.stream));
+    nv = 0;
+    for (int64_t i = 0; i < n; ++i)
+      if (!std::isnan(tmp[i])) {
+        for (int q = 0; q < 6; ++q) tmp[q * (size_t)n + nv] = tmp[q * (size_t)n + i];
+        ++nv;
+      }
+  }
+  *nout = nv + nh + (int64_t)keep.size();
   if (*nout > cap) throw Error(FOPIS_E_ARG, "output buffer too small: need " + std::to_string(*nout));
-  for (int q = 0; q < 6 && n > 0; ++q)
-    FOPIS_HIP(hipMemcpyAsync(h[q], out.c[q].p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  FOPIS_HIP(hipStreamSynchronize(c.stream));
-  for (int q = 0; q < 6 && nh > 0; ++q) std::memcpy(h[q] + n, b.host_inbox.c[q], nh * sizeof(double));
+  if (b.has_invalid) {
+    for (int q = 0; q < 6 && nv > 0; ++q) std::memcpy(h[q], tmp.data() + q * (size_t)n, nv * sizeof(double));
+  } else {
+    for (int q = 0; q < 6 && n > 0; ++q)
+      FOPIS_HIP(hipMemcpyAsync(h[q], out.c[q].p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    FOPIS_HIP(hipStreamSynchronize(c.stream));
+  }
+  for (int q = 0; q < 6 && nh > 0; ++q) std::memcpy(h[q] + nv, b.host_inbox.c[q], nh * sizeof(double));
   for (int q = 0; q < 6; ++q)
-    for (size_t k = 0; k < keep.size(); ++k) h[q][n + nh + (int64_t)k] = dv[q * (size_t)nd + keep[k]];
+    for (size_t k = 0; k < keep.size(); ++k) h[q][nv + nh + (int64_t)k] = dv[q * (size_t)nd + keep[k]];
 }
 
 void particles_get(Ctx& c, int s, int64_t cap, double* const* h, int64_t* n) {
@@ -1867,12 +1955,39 @@ void particles_velocity(Ctx& c) {
 }
 
 void particles_position(Ctx& c) {
-  Phase ph(c, "position");
+  {
+    Phase ph(c, "position");
+    FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), c.stream));
+    for (int s = 0; s < c.S; ++s) {
+      if (c.fp32) {
+        launch_tiles<float, M_POS>(c, s);
+        c.sp32[s].has_invalid = true;
+      } else {
+        launch_tiles<double, M_POS>(c, s);
+        c.sp64[s].has_invalid = true;
+      }
+    }
+  }
+  std::vector<int32_t> stat(kStat * c.S);
+  FOPIS_HIP(hipMemcpyAsync(stat.data(), c.status.p, stat.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
   for (int s = 0; s < c.S; ++s) {
     if (c.fp32)
-      launch_tiles<float, M_POS>(c, s);
+      c.sp32[s].absorbed += stat[kStat * s + 4];
     else
-      launch_tiles<double, M_POS>(c, s);
+      c.sp64[s].absorbed += stat[kStat * s + 4];
+  }
+}
+
+void particles_ledger(Ctx& c, int s, int64_t* out) {
+  if (c.fp32) {
+    out[0] = c.sp32[s].joined;
+    out[1] = c.sp32[s].absorbed;
+    out[2] = c.sp32[s].captured;
+  } else {
+    out[0] = c.sp64[s].joined;
+    out[1] = c.sp64[s].absorbed;
+    out[2] = c.sp64[s].captured;
   }
 }
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "fopis_guess_dataset_get", "fopis_guess_dataset_set", "fopis_guess_grad", "fopis_guess_train",
     "fopis_guess_eval", "fopis_guess_synth_boxes", "fopis_nn_conv3d", "fopis_nn_conv3d_backward",
     "fopis_first_solve_cycles", "fopis_moments", "fopis_guess_set_loss", "fopis_inject_maxwellian",
-    "fopis_emit_photoelectrons",
+    "fopis_emit_photoelectrons", "fopis_particle_ledger",
 ]
 
 
@@ -187,6 +187,13 @@ class Context:
         self._ok(self.L.fopis_get_particles(self.h, s, cap, *[_dp(a) for a in arrs], C.byref(n)))
         return dict(zip(("x", "y", "z", "vx", "vy", "vz"), [a[: n.value] for a in arrs]))
 
+    def load_maxwellian(self, s: int, n: int, seed: int, lo, hi, drift, sigma: float) -> None:
+        """Replace species s by n particles drawn on the device (uniform in [lo, hi), N(drift, sigma))."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (lo, hi, drift)]
+        self.L.fopis_load_maxwellian.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.POINTER(C.c_double),
+                                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double]
+        self._ok(self.L.fopis_load_maxwellian(self.h, s, n, seed, *[_dp(v) for v in a], sigma))
+
     def cell_keys(self, s: int) -> np.ndarray:
         cap = self.num_particles(s)
         out = np.empty(cap, np.int32)
@@ -266,6 +273,14 @@ class Context:
         self.L.fopis_moments.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         self._ok(self.L.fopis_moments(self.h, s, _dp(out)))
         return {"count": out[0], "mean_x": out[1:4], "mean_v": out[4:7], "var_v": out[7:10]}
+
+    def ledger(self, s: int) -> dict:
+        """Species s since the context was made: staged particles that joined,
+        particles absorbed at a domain face, particles captured by a conductor."""
+        out = np.zeros(3, np.int64)
+        self.L.fopis_particle_ledger.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
+        self._ok(self.L.fopis_particle_ledger(self.h, s, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {"joined": int(out[0]), "absorbed": int(out[1]), "captured": int(out[2])}
 
     def energy(self):
         out = np.zeros(2)

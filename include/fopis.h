@@ -4,8 +4,9 @@
  * The reference (hsimyu/fopis) has no foreign-function interface: its hot path
  * sits behind the C++ class surface of Grid/RootGrid/Field/Spacecraft. Each
  * entry point below replaces one of those methods and says which (file:line in
- * the reference). A C++ host facade (fopis_amd/csrc/host_grid.hpp, class
- * fopis::GpuRootGrid) keeps the reference's method names on top of this ABI.
+ * the reference). A C++ host facade (fopis_amd/csrc/host.cpp, class
+ * fopis::GpuRootGrid, C entry points in fopis_host.h) keeps the reference's
+ * method names on top of this ABI.
  *
  * Conventions
  *  - Every function returns 0 on success and a negative FOPIS_E_* code on
@@ -139,9 +140,11 @@ int fopis_emit_photoelectrons(fopis_ctx* ctx, int obj, int s, const double* shin
 /* Number of live particles of species s (Grid::getValidParticleNumber,
  * GridParticleTreatment.cpp:120-128). */
 int fopis_num_particles(fopis_ctx* ctx, int s, int64_t* n);
-/* Copies species s out in device order (tile-major), followed by the
- * particles staged for the next deposit (added from the host, then drawn on
- * the device). cap = buffer length. */
+/* Copies the live particles of species s out in device order (tile-major),
+ * followed by the particles staged for the next deposit (added from the
+ * host, then drawn on the device, those inside the domain). Particles a
+ * standalone position update absorbed are not copied. cap = buffer length;
+ * the count is that of fopis_num_particles. */
 int fopis_get_particles(fopis_ctx* ctx, int s, int64_t cap, double* x, double* y, double* z, double* vx,
                         double* vy, double* vz, int64_t* n);
 /* Stages new particles of species s (injected at the boundary or emitted by
@@ -150,9 +153,14 @@ int fopis_get_particles(fopis_ctx* ctx, int s, int64_t cap, double* x, double* y
  * Spacecraft::emitParticles (Spacecraft.cpp:887-897). */
 int fopis_add_particles(fopis_ctx* ctx, int s, int64_t n, const double* x, const double* y, const double* z,
                         const double* vx, const double* vy, const double* vz);
-/* Stable counting sort of every species by the reference cell key
- * ((i*(ny+1)+j)*(nz+1)+k with i = floor(x)+1, position.hpp:59-69); the order
- * is then what std::stable_sort by that key gives on the device order. */
+/* Stable sort of every species by cell inside its particle tiles (tiles of
+ * 8x8x8 cells, the unit the device keeps particles binned by): the order is
+ * what std::stable_sort gives on the device order with the key
+ * tile * 512 + ((i % 8) * 8 + j % 8) * 8 + k % 8, where (i, j, k) = floor(x, y, z)
+ * and tile = ((i / 8) * T1 + j / 8) * T2 + k / 8 (T = tiles per axis). Cells
+ * of one tile are contiguous and in the reference's (i, j, k) order
+ * (position.hpp:59-69); on a grid of one tile this is the reference cell-key
+ * order. A particle at the upper face (x = nx - 1) counts in the last cell. */
 int fopis_sort_by_cell(fopis_ctx* ctx);
 /* Reference cell keys of species s in device order. */
 int fopis_get_cell_keys(fopis_ctx* ctx, int s, int64_t cap, int32_t* keys, int64_t* n);
@@ -226,8 +234,14 @@ int fopis_set_object_charge_map(fopis_ctx* ctx, int obj, const double* charge_ma
 /* ---- measurement ------------------------------------------------------------- */
 
 /* Average device time (ms) of the named phase over the calls since the last
- * reset, measured with HIP events on the context's stream: "push_deposit",
- * "poisson", "efield", "bin". n = number of samples. */
+ * reset, measured with HIP events on the context's stream. Phases:
+ * "particles" (inbox upload, fused tile kernel, arrivals, capture and the
+ * window reduction of one deposit), inside it "tile" (the tile kernels of all
+ * species), "bin" (arrivals) and "reduce" (windows into rho and density);
+ * "poisson" (each solve), "poisson_first" (the learned guess, when on, plus
+ * the first solve of a step with objects), "guess" (network inference),
+ * "efield", "velocity" and "position" (the standalone updates).
+ * n = number of samples. */
 int fopis_phase_time(fopis_ctx* ctx, const char* phase, double* avg_ms, int64_t* n);
 int fopis_reset_timers(fopis_ctx* ctx);
 /* Poisson statistics: cycles and relres of the last solve, totals since the
@@ -245,6 +259,16 @@ int fopis_moments(fopis_ctx* ctx, int s, double* out);
  * step since fopis_reset_timers (the solve the learned guess starts; phase
  * "poisson_first" of fopis_phase_time times guess + that solve). */
 int fopis_first_solve_cycles(fopis_ctx* ctx, int64_t* total_cycles);
+/* Particle ledger of species s since the context was created: out[0]
+ * staged particles (added, injected or emitted) that joined the particle
+ * set inside the domain, out[1] particles absorbed at a domain face by a
+ * position update (grid.hpp:309-395), out[2] particles captured by a
+ * conductor (Spacecraft::distributeInnerParticleCharge, Spacecraft.cpp:
+ * 512-592). Between two calls the live count (fopis_num_particles) changes
+ * by exactly out[0] - out[1] - out[2]; set, load and sort calls replace the
+ * species and are outside that balance. The reference keeps no such count;
+ * its particle arrays are the ledger (Grid::getValidParticleNumber). */
+int fopis_particle_ledger(fopis_ctx* ctx, int s, int64_t* out);
 /* Blocks until the context's stream is idle. */
 int fopis_synchronize(fopis_ctx* ctx);
 

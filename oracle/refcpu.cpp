@@ -123,7 +123,11 @@ static Position genPosition(Species& s, double x0, double x1, double y0, double 
 
 // AmbientParticleType.cpp:85-93 — a fresh normal_distribution per call
 static void genVelocity(Species& s, const Units& u, double v[3]) {
+#if REFCPU_MUTANT == 6  // planted mistake: the injection's sqrt(2T/m) in place of sqrt(T/m)
+  const double dev = s.thermalVelocity(u);
+#else
   const double dev = s.deviation(u);
+#endif
   std::normal_distribution<> nx(s.drift[0], dev), ny(s.drift[1], dev), nz(s.drift[2], dev);
   v[0] = nx(s.mt_vx);
   v[1] = ny(s.mt_vy);
@@ -319,10 +323,17 @@ bool Spacecraft::isZsurfacePoint(const Position& p, int sign) const {
 // Spacecraft.cpp:594-613 (sgn = +1) and 945-970 (sgn = -1, emission)
 void Spacecraft::addX(const Position& p, int id, double q, double sgn) {
   double* t = &temporary_charge_map[(size_t)id * num_cmat];
+#if REFCPU_MUTANT == 1  // planted mistake for the pin tests: dy1 and dy2 swapped
+  t[cmatNumber(p.i, p.j, p.k)] += sgn * (q * p.dy1 * p.dz2);
+  t[cmatNumber(p.i, p.j + 1, p.k)] += sgn * (q * p.dy2 * p.dz2);
+  t[cmatNumber(p.i, p.j, p.k + 1)] += sgn * (q * p.dy1 * p.dz1);
+  t[cmatNumber(p.i, p.j + 1, p.k + 1)] += sgn * (q * p.dy2 * p.dz1);
+#else
   t[cmatNumber(p.i, p.j, p.k)] += sgn * (q * p.dy2 * p.dz2);
   t[cmatNumber(p.i, p.j + 1, p.k)] += sgn * (q * p.dy1 * p.dz2);
   t[cmatNumber(p.i, p.j, p.k + 1)] += sgn * (q * p.dy2 * p.dz1);
   t[cmatNumber(p.i, p.j + 1, p.k + 1)] += sgn * (q * p.dy1 * p.dz1);
+#endif
 }
 void Spacecraft::addY(const Position& p, int id, double q, double sgn) {
   double* t = &temporary_charge_map[(size_t)id * num_cmat];
@@ -689,7 +700,11 @@ void Grid::updateParticlePositionES() {
       p.x += p.vx;
       p.y += p.vy;
       p.z += p.vz;
+#if REFCPU_MUTANT == 5  // planted mistake: a particle exactly on the upper x face is absorbed
+      if (p.x < 0.0 || p.x >= (slx - dx)) p.isValid = 0;
+#else
       if (p.x < 0.0 || p.x > (slx - dx)) p.isValid = 0;
+#endif
       if (p.y < 0.0 || p.y > (sly - dx)) p.isValid = 0;
       if (p.z < 0.0 || p.z > (slz - dx)) p.isValid = 0;
     }
@@ -832,7 +847,11 @@ void Grid::updateDensity() {
     for (const auto& p : particles[s]) {
       if (!p.isValid) continue;
       const Position pos(p.x, p.y, p.z);
+#if REFCPU_MUTANT == 4  // planted mistake: the cell index without the glue offset along x
+      density[s][((size_t)(pos.i - 1) * dy_ + pos.j) * dz_ + pos.k] += size;
+#else
       density[s][((size_t)pos.i * dy_ + pos.j) * dz_ + pos.k] += size;
+#endif
     }
   }
 }
@@ -1088,7 +1107,11 @@ double Grid::particleEnergy() const {
   for (int s = 0; s < S; ++s) {
     double e = 0.0;
     for (const auto& p : particles[s])
+#if REFCPU_MUTANT == 2  // planted mistake: the vz term dropped
+      if (p.isValid) e += (p.vx * p.vx + p.vy * p.vy);
+#else
       if (p.isValid) e += (p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
+#endif
     res += 0.5 * sp[s].mass * e * sp[s].size;
   }
   return res;
@@ -1102,7 +1125,9 @@ double Grid::efieldEnergy() const {
       for (int k = 1; k < cz - 1; ++k) {
         if (i < cx - 2) e += std::pow(ex[((size_t)i * cy + j) * cz + k], 2);
         if (j < cy - 2) e += std::pow(ey[((size_t)i * (cy - 1) + j) * cz + k], 2);
+#if REFCPU_MUTANT != 3  // planted mistake (mutant 3): the ez edges dropped
         if (k < cz - 2) e += std::pow(ez[((size_t)i * cy + j) * (cz - 1) + k], 2);
+#endif
       }
   return 0.5 * u.eps0 * e;
 }

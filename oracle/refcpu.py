@@ -61,19 +61,27 @@ class Params(C.Structure):
 # field selectors (refcpu_capi.cpp fieldPtr)
 PHI, RHO, DENSITY, EX, EY, EZ, EXREF, EYREF, EZREF, RESIDUAL, ERROR = range(11)
 
-_lib = None
+_libs: dict = {}
 
 
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def mutant_path(n: int) -> str:
+    """The oracle built with planted mistake n (REFCPU_MUTANT in refcpu.cpp),
+    compiled on first use: the pin tests show their checks fail on it."""
+    path = os.path.join(HERE, "_mut", f"librefcpu_m{n}.so")
+    subprocess.run(["make", "-s", "-C", HERE, path], check=True)
+    return path
+
+
+def lib(path: str | None = None) -> C.CDLL:
+    path = path or LIB_PATH
+    if path not in _libs:
+        if path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         vp, i, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
         dp = C.POINTER(C.c_double)
         L.rc_create.restype = vp
@@ -135,8 +143,8 @@ def lib() -> C.CDLL:
         L.rc_params_size.restype = i64
         L.rc_set_threads.argtypes = [i]
         assert L.rc_params_size() == C.sizeof(Params), "Params layout mismatch"
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _dp(a: np.ndarray):
@@ -184,8 +192,8 @@ def params(nx, ny, nz, dx, dt, species_list, boundary="DDDDDD", static_b_nT=(0, 
 class Grid:
     """One oracle grid (the reference RootGrid on one rank)."""
 
-    def __init__(self, p: Params):
-        self.L = lib()
+    def __init__(self, p: Params, lib_path: str | None = None):
+        self.L = lib(lib_path)
         self.p = p
         self.h = self.L.rc_create(C.byref(p))
         if not self.h:

