@@ -185,6 +185,7 @@ int fopis_destroy(fopis_ctx* ctx) {
   timers_collect(ctx->c);
   hipStream_t st = ctx->c.stream;
   guess_free(ctx->c);
+  poisson_free(ctx->c);
   delete ctx;
   if (st) (void)hipStreamDestroy(st);
   return FOPIS_OK;

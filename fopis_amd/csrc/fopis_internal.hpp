@@ -159,6 +159,7 @@ struct PinSoa {
 template <class R>
 struct SpeciesBins {
   Soa<R> reg;            // tile regions
+  Soa<R> reg2;           // second buffer of the regions (the binning pass writes it, then they swap)
   DBuf<int64_t> start;   // [ntiles] region start
   DBuf<int32_t> capa;    // [ntiles] region capacity
   DBuf<int32_t> cnt;     // [ntiles] live particles in the region
@@ -271,6 +272,12 @@ struct Ctx {
   int64_t mg_first_cycles_total = 0;  // cycles of the first solve of each step
   double relres_last = 0;
   DBuf<int32_t> mg_state;     // [4] solve flag and cycle count on the device
+  DBuf<double> mg_part;       // per-block residual partials and the norms of the solve (fixed address)
+  hipGraphExec_t mg_graph = nullptr;  // one V-cycle + FP64 update + norm check, captured once
+  size_t mg_graph_esz = 0;    // element size the graph was captured for
+  double mg_graph_tol = 0.0;  // tolerance and cycle cap captured in its norm kernel
+  int mg_graph_cap = 0;
+  bool mg_use_graph = true;   // replay the captured cycle (FOPIS_MG_GRAPH=0 launches it kernel by kernel)
   DBuf<double> mg_r64;        // FP64 level-0 residual when the V-cycle is FP32
   int mg_hist[2] = {1, 1};    // cycles of the last two solves (batch size of the next)
   int mg_tail = 0;            // first level solved by the single-workgroup LDS kernel
@@ -283,6 +290,7 @@ struct Ctx {
   std::map<std::string, PhaseTimer> timers;
   bool timing = true;
   int timestep = 1;
+  bool tile_double_buffer = true;  // FOPIS_TILE_INPLACE=1 compacts the regions in place instead
 
   size_t id3(int i, int j, int k) const { return ((size_t)i * cy + j) * cz + k; }
 };
@@ -315,6 +323,7 @@ void fields_set(Ctx& c, int which, int s, const double* in);
 double fields_energy(Ctx& c);
 
 void poisson_init(Ctx& c);
+void poisson_free(Ctx& c);
 int poisson_solve(Ctx& c, double tol, int max_cycles, double* relres, const double* rhs_override);
 
 void objects_set(Ctx& c, int o, const uint8_t* mask, int ncmat, const int32_t* ijk, int fixed, double fixed_pot);

@@ -15,6 +15,7 @@
 // without contraction, so an FP64 push reproduces the reference bit for bit
 // on identical inputs.
 #include <cmath>
+#include <cstdlib>
 
 #include "fopis_internal.hpp"
 #include "radix.hpp"
@@ -40,6 +41,7 @@ struct TileArgs {
   double qm, q, bx, by, bz, boris, size;
   const double *ex, *ey, *ez;
   SoaPtr<R> reg;
+  SoaPtr<R> dst;   // regions the binning pass writes (the other buffer of a double-buffered set, or reg)
   const int64_t* start;
   const int32_t* capa;
   int32_t* cnt;
@@ -404,11 +406,18 @@ __device__ __forceinline__ unsigned long long fix_weight(double v) {
 // particles (and the staged new ones) in chunks, apply the requested stages,
 // accumulate charge and counts in LDS windows, and re-bin: particles staying
 // in the tile are compacted in place, the others go to the tile's outbox.
-template <class R, int MODE>
+// DB: the binning pass writes the tile's particles to the other buffer
+// (a.dst) and every wavefront reserves its slots with one LDS atomic per
+// iteration, so the waves of a block never wait for each other until the
+// end; otherwise particles are compacted in place (a.dst == a.reg), which
+// needs a block-wide scan with two barriers per iteration so that no slot is
+// written before it has been read.
+template <class R, int MODE, bool DB = false>
 __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   using SM = TileSmem<R, MODE>;
   __shared__ __attribute__((aligned(16))) char smem[SM::bytes];
   __shared__ int sh_rank[4 * kWaves];
+  __shared__ int s_res[2];  // DB: slots reserved so far in the region and in the outbox
   __shared__ int s_dir[(MODE & M_BIN) != 0 ? 27 : 1];  // outbox entries per direction
   R* sE = reinterpret_cast<R*>(smem);
   unsigned long long* sR = reinterpret_cast<unsigned long long*>(smem + SM::e_bytes);
@@ -445,6 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   }
   if constexpr ((MODE & M_BIN) != 0) {
     if (threadIdx.x < 27) s_dir[threadIdx.x] = 0;
+    if (threadIdx.x < 2) s_res[threadIdx.x] = 0;
   }
   __syncthreads();
 
@@ -703,17 +713,49 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         stay[u] = valid[u] && !far[u] && dir[u] == 13;
         move[u] = valid[u] && !far[u] && dir[u] != 13;
       }
-      // ranks in (u, thread) order; the scan's barriers also order all reads before the writes
       int rs[PPT], rm[PPT], ts[PPT], tm[PPT];
-      if constexpr (PPT == 1)
-        block_rank2(stay[0], move[0], rs[0], rm[0], ts[0], tm[0], sh_rank);
-      else
-        block_rank4(stay[0], move[0], stay[1], move[1], rs, rm, ts, tm, sh_rank);
+      if constexpr (DB) {
+        // slots of this wavefront: its stayers and movers in (u, lane) order
+        // after one reservation each in the region and the outbox
+        const int lane = threadIdx.x & 63;
+        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        unsigned long long ms[PPT], mm[PPT];
+        int ns = 0, nm = 0;
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+          ms[u] = __ballot(stay[u]);
+          mm[u] = __ballot(move[u]);
+          rs[u] = ns + __popcll(ms[u] & lt);
+          rm[u] = nm + __popcll(mm[u] & lt);
+          ns += __popcll(ms[u]);
+          nm += __popcll(mm[u]);
+        }
+        int bs = 0, bm = 0;
+        if (lane == 0) {
+          if (ns) bs = atomicAdd(&s_res[0], ns);
+          if (nm) bm = atomicAdd(&s_res[1], nm);
+        }
+        bs = __builtin_amdgcn_readfirstlane(bs);
+        bm = __builtin_amdgcn_readfirstlane(bm);
+#pragma unroll
+        for (int u = 0; u < PPT; ++u) {
+          rs[u] += bs;
+          rm[u] += bm;
+          ts[u] = 0;
+          tm[u] = 0;
+        }
+      } else {
+        // ranks in (u, thread) order; the scan's barriers also order all reads before the writes
+        if constexpr (PPT == 1)
+          block_rank2(stay[0], move[0], rs[0], rm[0], ts[0], tm[0], sh_rank);
+        else
+          block_rank4(stay[0], move[0], stay[1], move[1], rs, rm, ts, tm, sh_rank);
+      }
       // Every lane issues the same seven stores per particle slot (a lane with
       // nothing to keep writes to the sink): with a fixed count of memory
       // operations per iteration, the next particles' loads are waited for
       // without waiting for this iteration's stores.
-      int wb = wptr, ob0 = optr;
+      int wb = DB ? 0 : wptr, ob0 = DB ? 0 : optr;
 #pragma unroll
       for (int u = 0; u < PPT; ++u) {
         int kind = 3;  // 0 tile region, 1 outbox, 2 stray list, 3 sink
@@ -748,7 +790,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-          __attribute__((address_space(1))) R* d = kind == 0 ? sgpr_ptr(a.reg.c[c])
+          __attribute__((address_space(1))) R* d = kind == 0 ? sgpr_ptr(a.dst.c[c])
                            : (kind == 1 ? sgpr_ptr(a.out.c[c]) : (kind == 2 ? sgpr_ptr(a.stray.c[c]) : sgpr_ptr(a.sink)));
           d[at] = p[u][c];
         }
@@ -774,10 +816,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     }
   }
 
+  if constexpr (DB) __syncthreads();  // every wavefront's reservations are in
   if constexpr ((MODE & M_BIN) != 0) {
     if (threadIdx.x == 0) {
-      a.cnt[t] = wptr;
-      a.ocnt[t] = optr;
+      a.cnt[t] = DB ? min(s_res[0], rcap) : wptr;
+      a.ocnt[t] = DB ? min(s_res[1], ocap) : optr;
     }
   }
   if constexpr (SM::kR || SM::kD || (MODE & M_BIN) != 0) __syncthreads();
@@ -852,7 +895,7 @@ __global__ __launch_bounds__(kBlock) void k_arrive(TileArgs<R> a) {
         const int pos = off + __popcll(bm & lt);
         if (pos < cap) {
 #pragma unroll
-          for (int c = 0; c < 6; ++c) a.reg.c[c][base + pos] = p[c];
+          for (int c = 0; c < 6; ++c) a.dst.c[c][base + pos] = p[c];
         } else {
           put_stray(a, p);
         }
@@ -1372,6 +1415,7 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.ey = c.eref.p + c.nnode;
   a.ez = c.eref.p + 2 * c.nnode;
   a.reg = ptrs(b.reg);
+  a.dst = a.reg;
   a.start = b.start.p;
   a.capa = b.capa.p;
   a.cnt = b.cnt.p;
@@ -1453,7 +1497,10 @@ void bin_flat(Ctx& c, int s, Soa<R>& flat, int64_t n, bool key_cell) {
     FOPIS_HIP(hipMemcpyAsync(&tot[0], b.start.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     FOPIS_HIP(hipMemcpyAsync(&tot[1], b.ostart.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     FOPIS_HIP(hipStreamSynchronize(st));
-    if ((size_t)tot[0] != b.reg.size()) b.reg.alloc(std::max<int64_t>(tot[0], 1));
+    if ((size_t)tot[0] != b.reg.size()) {
+      b.reg.alloc(std::max<int64_t>(tot[0], 1));
+      b.reg2.release();  // sized again by the next binning pass that needs it
+    }
     if ((size_t)tot[1] != b.out.size()) {
       b.out.alloc(std::max<int64_t>(tot[1], 1));
       b.odir.alloc(std::max<int64_t>(tot[1], 1));
@@ -1575,6 +1622,18 @@ void launch_tiles(Ctx& c, int s) {
   FOPIS_HIP(hipGetLastError());
 }
 
+// The binning pass of species s into its second region buffer (allocated on
+// first use, the size of the first); the buffers swap once the arrivals are in.
+template <class R, int MODE>
+void launch_tiles_db(Ctx& c, int s) {
+  auto& b = bins<R>(c)[s];
+  if (b.reg2.size() != b.reg.size()) b.reg2.alloc(b.reg.size());
+  TileArgs<R> a = tile_args<R>(c, s);
+  a.dst = ptrs(b.reg2);
+  k_tile<R, MODE, true><<<c.ntiles, kBlock, 0, c.stream>>>(a);
+  FOPIS_HIP(hipGetLastError());
+}
+
 template <class R>
 void after_bin(Ctx& c) {
   std::vector<int32_t> stat(kStat * c.S);
@@ -1618,20 +1677,32 @@ void deposit_bin(Ctx& c, bool push) {
   FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), st));
   for (int o = 0; o < c.nobj; ++o)
     if (c.obj[o].absorbed.p) FOPIS_HIP(hipMemsetAsync(c.obj[o].absorbed.p, 0, c.S * sizeof(unsigned long long), st));
+  const bool db = c.tile_double_buffer;
   {
     Phase ph(c, "tile");
     for (int s = 0; s < c.S; ++s) {
-      if (push)
-        launch_tiles<R, M_VEL | M_POS | M_DEP | M_BIN>(c, s);
-      else
-        launch_tiles<R, M_DEP | M_BIN>(c, s);
+      if (db) {
+        if (push)
+          launch_tiles_db<R, M_VEL | M_POS | M_DEP | M_BIN>(c, s);
+        else
+          launch_tiles_db<R, M_DEP | M_BIN>(c, s);
+      } else {
+        if (push)
+          launch_tiles<R, M_VEL | M_POS | M_DEP | M_BIN>(c, s);
+        else
+          launch_tiles<R, M_DEP | M_BIN>(c, s);
+      }
     }
   }
   {
     Phase ph(c, "bin");
     for (int s = 0; s < c.S; ++s) {
+      auto& b = bins<R>(c)[s];
       TileArgs<R> a = tile_args<R>(c, s);
+      if (db) a.dst = ptrs(b.reg2);
       k_arrive<R><<<c.ntiles, kBlock, 0, st>>>(a);
+      if (db)
+        for (int q = 0; q < 6; ++q) b.reg.c[q].swap(b.reg2.c[q]);
     }
     FOPIS_HIP(hipGetLastError());
   }
@@ -1649,6 +1720,8 @@ void deposit_bin(Ctx& c, bool push) {
 }  // namespace
 
 void particles_init(Ctx& c) {
+  const char* ip = std::getenv("FOPIS_TILE_INPLACE");
+  c.tile_double_buffer = !(ip && ip[0] == '1');
   if (c.fp32)
     c.sp32.resize(c.S);
   else

@@ -17,6 +17,7 @@
 // one single-workgroup kernel per cycle.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 
 #include "fopis_internal.hpp"
@@ -698,6 +699,8 @@ dim3 grid_full(const MgLevel& m) { return dim3(div_up(m.n[2], 64), m.n[1], m.n[0
 dim3 grid_full4(const MgLevel& m) { return dim3(div_up(m.n[2], 64), div_up(m.n[1], kRows), m.n[0]); }
 dim3 grid_half4(const MgLevel& m) { return dim3(div_up((m.n[2] + 1) / 2, 64), div_up(m.n[1], kRows), m.n[0]); }
 
+constexpr int kResidBlocks = 2048;  // workgroups of the level-0 residual and update passes
+
 // Levels with at least this many unknowns form their residual in a pass of
 // its own before restricting it (below, one launch forming it on the fly costs less).
 constexpr int64_t kSplitRestrict = (int64_t)1 << 20;
@@ -826,10 +829,9 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   const double* rho = rho0 + off;
   P* r0 = R_<P>(m0);
   const double rc = 1.0 / c.d.eps0;  // pow(dx, 2) / eps0 with dx = 1
-  const int nb = 2048;  // workgroups of the level-0 residual passes (8 per CU)
-  c.red.alloc(std::max<size_t>(c.red.n, 2 * nb + 4));
-  double* part = c.red.p;
-  double* dst = c.red.p + 2 * nb;
+  const int nb = kResidBlocks;  // workgroups of the level-0 residual passes (8 per CU)
+  double* part = c.mg_part.p;
+  double* dst = c.mg_part.p + 2 * nb;
   int* st = c.mg_state.p;
   const int* done = st;
   if (c.mg_tail_esz != sizeof(P)) {
@@ -853,11 +855,43 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   int batch = std::max(1, std::min(c.mg_hist[0], c.mg_hist[1]));
   int32_t hs[2];
   double hd[3];
+  // One cycle: the V-cycle, the FP64 update and the norm check. Its ~30
+  // launches are captured once into a graph of kernel nodes (every pointer
+  // and size in them is fixed for the context; the tolerance and the cycle
+  // cap are arguments of the norm kernel, so the graph is rebuilt when they
+  // change) and replayed per cycle.
+  auto cycle = [&]() {
+    vcycle<P>(c, 0, done);
+    k_update<P><<<nb, 256, 0, c.stream>>>(phi, E_<P>(m0), r64, r0, L0, part, done);
+    k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, 1, 1);
+  };
+  if (c.mg_use_graph && (!c.mg_graph || c.mg_graph_esz != sizeof(P) || c.mg_graph_tol != tol ||
+                         c.mg_graph_cap != max_cycles)) {
+    if (c.mg_graph) {
+      FOPIS_HIP(hipGraphExecDestroy(c.mg_graph));
+      c.mg_graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    FOPIS_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    cycle();
+    const hipError_t ec = hipStreamEndCapture(c.stream, &g);
+    if (ec != hipSuccess || !g) throw Error(FOPIS_E_DEVICE, std::string("V-cycle capture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&c.mg_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      c.mg_graph = nullptr;
+      throw Error(FOPIS_E_DEVICE, std::string("V-cycle graph: ") + hipGetErrorString(ei));
+    }
+    c.mg_graph_esz = sizeof(P);
+    c.mg_graph_tol = tol;
+    c.mg_graph_cap = max_cycles;
+  }
   for (;;) {
     for (int q = 0; q < batch; ++q) {
-      vcycle<P>(c, 0, done);
-      k_update<P><<<nb, 256, 0, c.stream>>>(phi, E_<P>(m0), r64, r0, L0, part, done);
-      k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, 1, 1);
+      if (c.mg_use_graph)
+        FOPIS_HIP(hipGraphLaunch(c.mg_graph, c.stream));
+      else
+        cycle();
     }
     FOPIS_HIP(hipMemcpyAsync(hs, st, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     FOPIS_HIP(hipStreamSynchronize(c.stream));
@@ -946,6 +980,9 @@ void poisson_init(Ctx& c) {
   }
   c.mg_state.alloc(4);
   FOPIS_HIP(hipMemsetAsync(c.mg_state.p, 0, 4 * sizeof(int32_t), c.stream));
+  c.mg_part.alloc(2 * kResidBlocks + 4);
+  const char* g = std::getenv("FOPIS_MG_GRAPH");
+  c.mg_use_graph = !(g && g[0] == '0');
   int dev = 0;
   FOPIS_HIP(hipGetDevice(&dev));
   hipDeviceProp_t pr;
@@ -954,6 +991,11 @@ void poisson_init(Ctx& c) {
   c.mg_tail_esz = 0;
   c.mg_hist[0] = c.mg_hist[1] = 1;
   FOPIS_HIP(hipStreamSynchronize(c.stream));
+}
+
+void poisson_free(Ctx& c) {
+  if (c.mg_graph) (void)hipGraphExecDestroy(c.mg_graph);
+  c.mg_graph = nullptr;
 }
 
 int poisson_solve(Ctx& c, double tol, int max_cycles, double* relres, const double* rhs_override) {
