@@ -6,8 +6,10 @@ step against invariants that hold whatever the particle order or the seed:
   that joined, minus those absorbed at a domain face, minus those captured by
   the conductor (RootGrid.cpp:82-143, grid.hpp:309-395,
   Spacecraft.cpp:512-592);
-* density (RootGridField.cpp:1129-1150): its sum is the live count times the
-  superparticle size, exactly;
+* density (RootGridField.cpp:1129-1150): its sum is the superparticle size
+  times the live count plus the particles the conductor captured in this
+  step, exactly (updateDensity runs before updateRho captures them,
+  RootGrid.cpp:115-121);
 * charge (RootGridField.cpp:1153-1214): each species' deposit sums to its
   superparticle charge times its live count (FP64 to 1e-12, FP32 to 2e-7
   relative), and rho[0] exceeds the species sum by exactly the conductor's
@@ -56,7 +58,7 @@ def _check_step(h, before, after, fp32):
         assert after["n"][s] == before["n"][s] + d["joined"] - d["absorbed"] - d["captured"], (s, before, after)
         sd = h.desc.species[s]
         dens = ctx.get_field(F.DENSITY, s)
-        assert dens.sum() == after["n"][s] * sd.size
+        assert dens.sum() == (after["n"][s] + d["captured"]) * sd.size
         rho = ctx.get_field(F.RHO, s + 1)
         want = sd.charge * sd.size * after["n"][s]
         tol = 2e-7 if fp32 else 1e-12
