@@ -66,19 +66,29 @@ static void check_desc(const fopis_desc& d) {
 
 // With the learned guess on, the first Poisson solve of a step starts from
 // the network's potential instead of the previous step's.
-static void update_rho(Ctx& c, bool push) {
+// In the fused step (carry), the two solves of a step with conductors start
+// from the warm starts objects_carry_* forms.
+static void update_rho(Ctx& c, bool push, bool carry = false) {
   particles_deposit_bin(c, push);
   objects_sum_whole_charge(c);
   if (c.nobj > 0) {
     double rr = 0.0;
-    Phase ph(c, "poisson_first");  // guess (when on) + the step's first solve
-    if (guess_mode(c) == 1) guess_apply(c);
-    poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+    {
+      Phase ph(c, "poisson_first");  // guess (when on) + the step's first solve
+      if (guess_mode(c) == 1)
+        guess_apply(c);
+      else if (carry)
+        objects_carry_first(c);
+      poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+    }
     c.mg_first_cycles_total += c.mg_cycles_last;
+    c.relres0_first += c.relres0_last;
+    ++c.n_first;
     // the object charge is redistributed from this potential: it must have converged
     if (!(rr <= c.d.poisson_tol))
       throw Error(FOPIS_E_NOCONV, "first Poisson solve stopped at relres " + std::to_string(rr));
     objects_redistribute(c);
+    if (carry) objects_carry_second(c);
   }
 }
 
@@ -308,15 +318,22 @@ int fopis_main_loop_es(fopis_ctx* ctx) {
   return guarded([&] {
     Ctx& c = ctx->c;
     objects_reset_step(c);
-    update_rho(c, true);
+    update_rho(c, true, true);
     double rr = 0.0;
     if (c.nobj == 0) {
-      Phase ph(c, "poisson_first");
-      if (guess_mode(c) == 1) guess_apply(c);
-      poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+      {
+        Phase ph(c, "poisson_first");
+        if (guess_mode(c) == 1) guess_apply(c);
+        poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+      }
       c.mg_first_cycles_total += c.mg_cycles_last;
+      c.relres0_first += c.relres0_last;
+      ++c.n_first;
     } else {
       poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
+      c.relres0_other += c.relres0_last;
+      ++c.n_other;
+      objects_carry_end(c);
     }
     if (!(rr <= c.d.poisson_tol)) throw Error(FOPIS_E_NOCONV, "Poisson solve stopped at relres " + std::to_string(rr));
     fields_efield(c);
@@ -346,6 +363,17 @@ int fopis_energy(fopis_ctx* ctx, double* out) {
     if (!out) throw Error(FOPIS_E_ARG, "null output");
     out[0] = particles_energy(ctx->c);
     out[1] = fields_energy(ctx->c);
+  });
+}
+
+int fopis_solver_start_residuals(fopis_ctx* ctx, double* out) {
+  NEED_CTX
+  return guarded([&] {
+    if (!out) throw Error(FOPIS_E_ARG, "null output");
+    const Ctx& c = ctx->c;
+    out[0] = c.relres0_last;
+    out[1] = c.n_first ? c.relres0_first / (double)c.n_first : 0.0;
+    out[2] = c.n_other ? c.relres0_other / (double)c.n_other : 0.0;
   });
 }
 
@@ -399,6 +427,14 @@ int fopis_get_object_state(fopis_ctx* ctx, int obj, double* potential, double* t
   });
 }
 
+int fopis_object_unattributed(fopis_ctx* ctx, int obj, int64_t* out) {
+  NEED_CTX
+  return guarded([&] {
+    if (obj < 0 || obj >= ctx->c.nobj || !out) throw Error(FOPIS_E_ARG, "bad argument");
+    objects_unattributed(ctx->c, obj, out);
+  });
+}
+
 int fopis_set_object_charge_map(fopis_ctx* ctx, int obj, const double* charge_map) {
   NEED_CTX
   return guarded([&] {
@@ -431,6 +467,8 @@ int fopis_reset_timers(fopis_ctx* ctx) {
     ctx->c.mg_cycles_total = 0;
     ctx->c.mg_solves_total = 0;
     ctx->c.mg_first_cycles_total = 0;
+    ctx->c.relres0_first = ctx->c.relres0_other = 0.0;
+    ctx->c.n_first = ctx->c.n_other = 0;
   });
 }
 

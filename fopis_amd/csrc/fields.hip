@@ -280,13 +280,17 @@ void fields_set(Ctx& c, int which, int s, const double* in) {
   FOPIS_HIP(hipStreamSynchronize(c.stream));
 }
 
+void fields_energy_queue(Ctx& c, double* part, double* sum) {
+  k_efield_energy<<<kEnergyBlocks, 256, 0, c.stream>>>(c.phi.p, c.cx, c.cy, c.cz, part);
+  k_serial_sum<<<1, 1, 0, c.stream>>>(part, kEnergyBlocks, sum);
+  FOPIS_HIP(hipGetLastError());
+}
+
 double fields_energy(Ctx& c) {
-  const int nb = 1024;
-  c.red.alloc(std::max<size_t>(c.red.n, nb + 1));
-  k_efield_energy<<<nb, 256, 0, c.stream>>>(c.phi.p, c.cx, c.cy, c.cz, c.red.p);
-  k_serial_sum<<<1, 1, 0, c.stream>>>(c.red.p, nb, c.red.p + nb);
+  c.red.alloc(std::max<size_t>(c.red.n, kEnergyBlocks + 1));
+  fields_energy_queue(c, c.red.p, c.red.p + kEnergyBlocks);
   double e = 0.0;
-  FOPIS_HIP(hipMemcpyAsync(&e, c.red.p + nb, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipMemcpyAsync(&e, c.red.p + kEnergyBlocks, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   FOPIS_HIP(hipStreamSynchronize(c.stream));
   return 0.5 * c.d.eps0 * e;
 }

@@ -87,20 +87,37 @@ struct DBuf {
   }
 };
 
-// Six SoA component arrays of one particle set.
+// Six SoA component arrays of one particle set, held in one allocation:
+// component q starts q * size() elements after component 0, so a kernel can
+// address every component of any set from one base and one stride.
 template <class R>
 struct Soa {
-  DBuf<R> c[6];
+  struct Comp {
+    R* p = nullptr;
+    size_t n = 0;
+  };
+  DBuf<R> blk;
+  Comp c[6];
   void alloc(size_t n) {
-    for (auto& a : c) a.alloc(n);
+    if (n == c[0].n && blk.p) return;
+    blk.alloc(6 * n);
+    for (int q = 0; q < 6; ++q) {
+      c[q].p = n ? blk.p + (size_t)q * n : nullptr;
+      c[q].n = n;
+    }
   }
   void release() {
-    for (auto& a : c) a.release();
+    blk.release();
+    for (auto& a : c) a = Comp{};
   }
   size_t size() const { return c[0].n; }
   // grow-only: keeps the buffers when they already hold m elements
   void reserve(size_t m) {
     if (size() < m) alloc(m);
+  }
+  void swap(Soa& o) {
+    blk.swap(o.blk);
+    std::swap(c, o.c);
   }
 };
 
@@ -271,6 +288,12 @@ struct Ctx {
   int64_t mg_cycles_total = 0, mg_solves_total = 0;
   int64_t mg_first_cycles_total = 0;  // cycles of the first solve of each step
   double relres_last = 0;
+  double relres0_last = 0;     // relative residual the last solve started from
+  double relres0_first = 0, relres0_other = 0;  // sums over the first / the other solves of steps
+  int64_t n_first = 0, n_other = 0;
+  DBuf<double> psi;            // conductor response of the last step: phi after minus phi before the charge
+  int psi_step = -1;           // step that formed psi (-1: none)
+  bool carry = true;           // warm starts from psi (FOPIS_CARRY=0: the previous phi for both solves)
   DBuf<int32_t> mg_state;     // [4] solve flag and cycle count on the device
   DBuf<double> mg_part;       // per-block residual partials and the norms of the solve (fixed address)
   hipGraphExec_t mg_graph = nullptr;  // one V-cycle + FP64 update + norm check, captured once
@@ -312,6 +335,12 @@ void particles_deposit_bin(Ctx& c, bool push);
 void particles_sort_by_cell(Ctx& c);
 void particles_cell_keys(Ctx& c, int s, int64_t cap, int32_t* keys, int64_t* n);
 double particles_energy(Ctx& c);
+// Energy reductions: kEnergyBlocks partials per quantity, summed in a fixed order.
+constexpr int kEnergyBlocks = 1024;
+// Queues sum v^2 of every species into sums[s] (part: S * kEnergyBlocks doubles).
+void particles_energy_queue(Ctx& c, double* part, double* sums);
+// 1/2 sum_s m_s size_s v2[s].
+double particles_energy_from(const Ctx& c, const double* v2);
 void particles_moments(Ctx& c, int s, double* out);
 void particles_ledger(Ctx& c, int s, int64_t* out);
 
@@ -321,6 +350,8 @@ void fields_reduce_windows(Ctx& c, bool do_rho, bool do_dens);
 void fields_get(Ctx& c, int which, int s, double* out);
 void fields_set(Ctx& c, int which, int s, const double* in);
 double fields_energy(Ctx& c);
+// Queues the sum of squared edge fields into *sum (part: kEnergyBlocks doubles).
+void fields_energy_queue(Ctx& c, double* part, double* sum);
 
 void poisson_init(Ctx& c);
 void poisson_free(Ctx& c);
@@ -332,6 +363,14 @@ void objects_set_cmatrix(Ctx& c, int o, const double* C);
 void objects_get_cmatrix(Ctx& c, int o, double* C);
 void objects_sum_whole_charge(Ctx& c);
 void objects_redistribute(Ctx& c);
+// Superparticles of each species captured in the last deposit whose entry face was not found.
+void objects_unattributed(Ctx& c, int o, int64_t* out);
+// Warm starts of the two solves of a step with conductors from the last
+// step's conductor response psi (phi after the charge redistribution minus
+// phi before it): first solve from phi - psi, second from phi + psi.
+void objects_carry_first(Ctx& c);
+void objects_carry_second(Ctx& c);
+void objects_carry_end(Ctx& c);
 void objects_add_charge(Ctx& c, int o, const double* dq, const double* dcur);
 void objects_state(Ctx& c, int o, double* pot, double* q, double* cur, double* cmap);
 void objects_set_charge_map(Ctx& c, int o, const double* cmap);

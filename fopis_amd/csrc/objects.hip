@@ -259,12 +259,12 @@ void objects_set(Ctx& c, int o, const uint8_t* mask, int ncmat, const int32_t* i
   ob.colsum.alloc(std::max(ncmat, 1));
   ob.tmp_charge.alloc(std::max<size_t>((size_t)c.S * ncmat, 1));
   ob.charge_map.alloc(std::max<size_t>((size_t)c.S * ncmat, 1));
-  ob.absorbed.alloc(c.S);
+  ob.absorbed.alloc(2 * (size_t)c.S);  // [S] captured, then [S] captured with no entry face found
   ob.scal.alloc(4 + std::max(ncmat, 1));
   FOPIS_HIP(hipMemsetAsync(ob.C.p, 0, ob.C.n * sizeof(double), st));
   FOPIS_HIP(hipMemsetAsync(ob.tmp_charge.p, 0, ob.tmp_charge.n * sizeof(double), st));
   FOPIS_HIP(hipMemsetAsync(ob.charge_map.p, 0, ob.charge_map.n * sizeof(double), st));
-  FOPIS_HIP(hipMemsetAsync(ob.absorbed.p, 0, c.S * sizeof(unsigned long long), st));
+  FOPIS_HIP(hipMemsetAsync(ob.absorbed.p, 0, 2 * c.S * sizeof(unsigned long long), st));
   FOPIS_HIP(hipMemsetAsync(ob.scal.p, 0, ob.scal.n * sizeof(double), st));
   ob.current.assign(c.S, 0.0);
   ob.host_current_add.assign(c.S, 0.0);
@@ -575,7 +575,7 @@ void objects_reset_step(Ctx& c) {
   for (int o = 0; o < c.nobj; ++o) {
     ObjectDev& ob = c.obj[o];
     if (!ob.defined) continue;
-    FOPIS_HIP(hipMemsetAsync(ob.absorbed.p, 0, c.S * sizeof(unsigned long long), c.stream));
+    FOPIS_HIP(hipMemsetAsync(ob.absorbed.p, 0, 2 * c.S * sizeof(unsigned long long), c.stream));
     ob.host_current_add = ob.host_current_pending;
     ob.host_current_pending.assign(c.S, 0.0);
   }
@@ -638,11 +638,73 @@ void objects_state(Ctx& c, int o, double* pot, double* q, double* cur, double* c
     FOPIS_HIP(hipMemcpy(cmap, ob.charge_map.p, (size_t)c.S * ob.ncmat * sizeof(double), hipMemcpyDeviceToHost));
 }
 
+void objects_unattributed(Ctx& c, int o, int64_t* out) {
+  ObjectDev& ob = c.obj[o];
+  std::vector<unsigned long long> ab(2 * (size_t)c.S, 0ull);
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  if (ob.absorbed.p)
+    FOPIS_HIP(hipMemcpy(ab.data(), ob.absorbed.p, 2 * c.S * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for (int s = 0; s < c.S; ++s) out[s] = (int64_t)ab[c.S + s];
+}
+
 void objects_set_charge_map(Ctx& c, int o, const double* cmap) {
   ObjectDev& ob = c.obj[o];
   if (ob.ncmat <= 0) return;
   FOPIS_HIP(hipStreamSynchronize(c.stream));
   FOPIS_HIP(hipMemcpy(ob.charge_map.p, cmap, (size_t)c.S * ob.ncmat * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// ---- warm starts carried over between steps
+//
+// The first solve of a step has the deposited charge alone on its right-hand
+// side, the second adds the conductor's redistributed charge; the previous
+// step's final potential carries that step's conductor response psi, which
+// the first solve must remove and the second must add back. Starting the
+// first solve from phi - psi and the second from (its first potential) + psi
+// leaves each with the change of one step to correct. Only the starting
+// points change: every solve still iterates to the same residual.
+__global__ void k_carry_first(double* phi, const double* psi, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) phi[i] -= psi[i];
+}
+
+// psi <- phi (the first solve's potential); phi += old psi when `add`
+__global__ void k_carry_second(double* phi, double* psi, int64_t n, int add) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double f = phi[i];
+  if (add) phi[i] = f + psi[i];
+  psi[i] = f;
+}
+
+// psi <- phi - psi (the final potential minus the first solve's)
+__global__ void k_carry_end(const double* phi, double* psi, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) psi[i] = phi[i] - psi[i];
+}
+
+void objects_carry_first(Ctx& c) {
+  if (!c.carry || c.psi_step != c.timestep - 1 || !c.psi.p) return;
+  const int64_t n = (int64_t)c.nnode;
+  k_carry_first<<<div_up(n, 256), 256, 0, c.stream>>>(c.phi.p, c.psi.p, n);
+  FOPIS_HIP(hipGetLastError());
+}
+
+void objects_carry_second(Ctx& c) {
+  if (!c.carry) return;
+  const int64_t n = (int64_t)c.nnode;
+  const bool add = c.psi_step == c.timestep - 1 && c.psi.p;
+  c.psi.alloc(c.nnode);
+  k_carry_second<<<div_up(n, 256), 256, 0, c.stream>>>(c.phi.p, c.psi.p, n, add ? 1 : 0);
+  FOPIS_HIP(hipGetLastError());
+}
+
+void objects_carry_end(Ctx& c) {
+  if (!c.carry || !c.psi.p) return;
+  const int64_t n = (int64_t)c.nnode;
+  k_carry_end<<<div_up(n, 256), 256, 0, c.stream>>>(c.phi.p, c.psi.p, n);
+  FOPIS_HIP(hipGetLastError());
+  c.psi_step = c.timestep;
 }
 
 }  // namespace fopis

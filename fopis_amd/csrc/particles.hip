@@ -30,6 +30,7 @@ struct ObjArgs {
   const int32_t* node_cmat;      // [(nx+2)(ny+2)(nz+2)] -> cmat index or -1
   double* tmp;                   // temporary charge map of this species
   unsigned long long* absorbed;  // absorbed superparticles of this species
+  unsigned long long* lost;      // of those, the ones whose entry face was not found
 };
 
 template <class R>
@@ -102,6 +103,7 @@ struct Cond {
   const int32_t* node_cmat;
   double* tmp;
   unsigned long long* absorbed;
+  unsigned long long* lost;
   __device__ bool valid_cell(int i, int j, int k) const {
     return i >= 0 && i < cx - 1 && j >= 0 && j < cy - 1 && k >= 0 && k < cz - 1;
   }
@@ -238,19 +240,23 @@ __device__ void absorb(const Cond& cd, double x, double y, double z, double vx, 
     n = 1;
   }
   const int is = vx > 0.0 ? 1 : -1, js = vy > 0.0 ? 1 : -1, ks = vz > 0.0 ? 1 : -1;
-  for (int q = 0; q < n; ++q) {
+  bool found = false;
+  for (int q = 0; q < n && !found; ++q) {
     const DPos& c = ord[q] == 0 ? xc : (ord[q] == 1 ? yc : zc);
     if (cd.xpoint(c, is)) {
       cd.addx(c);
-      break;
+      found = true;
     } else if (cd.ypoint(c, js)) {
       cd.addy(c);
-      break;
+      found = true;
     } else if (cd.zpoint(c, ks)) {
       cd.addz(c);
-      break;
+      found = true;
     }
   }
+  // the reference reports "Surface cannot detect" and still counts the
+  // particle in the current (Spacecraft.cpp:573-589); counted here instead
+  if (!found) atomicAdd(cd.lost, 1ull);
   atomicAdd(cd.absorbed, 1ull);
 }
 
@@ -262,7 +268,7 @@ __global__ void k_capture(const double* rec, int n, int cx, int cy, int cz, doub
   const double* r = rec + (size_t)8 * i;
   const int o = (int)r[6];
   const ObjArgs oa = o == 0 ? o0 : (o == 1 ? o1 : (o == 2 ? o2 : o3));
-  Cond cd{cx, cy, cz, o, q, mask, oa.node_cmat, oa.tmp, oa.absorbed};
+  Cond cd{cx, cy, cz, o, q, mask, oa.node_cmat, oa.tmp, oa.absorbed, oa.lost};
   absorb(cd, r[0], r[1], r[2], r[3], r[4], r[5]);
 }
 
@@ -390,6 +396,11 @@ struct TileSmem {
 // 5.7e-14 of a particle's charge.
 constexpr double kFix = 17592186044416.0;
 constexpr int kFixMaxParticles = 1 << 19;
+// FP32 particles use 2^31 units per unit weight: a weight's scaled value fits
+// 32 bits, and a node's signed 64-bit sum holds 2^32 particles' worth (more
+// than a tile launch can see), so FP32 tiles always deposit in LDS. The
+// quantum 2^-31 is below the FP32 weights' own rounding.
+constexpr float kFix32 = 2147483648.0f;
 
 // round(v * kFix) for a CIC weight v in [0, 1] (the value of an exact scaling
 // rounded to the nearest integer, ties to even). v * kFix + 2^52 lies in
@@ -479,6 +490,15 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   const int64_t ost = ((MODE & M_BIN) != 0) ? a.ostart[t] : 0;
   const int rcap = a.capa[t];
   const int64_t sink_at = ((int64_t)t * kBlock + threadIdx.x) & (kSink - 1);
+  // destinations of the binning stores (byte addresses and component strides)
+  const uint64_t dst_b = reinterpret_cast<uint64_t>(a.dst.c[0]);
+  const uint64_t dst_s = reinterpret_cast<uint64_t>(a.dst.c[1]) - dst_b;
+  const uint64_t out_b = reinterpret_cast<uint64_t>(a.out.c[0]);
+  const uint64_t out_s = reinterpret_cast<uint64_t>(a.out.c[1]) - out_b;
+  const uint64_t str_b = reinterpret_cast<uint64_t>(a.stray.c[0]);
+  const uint64_t str_s = reinterpret_cast<uint64_t>(a.stray.c[1]) - str_b;
+  const uint64_t snk_b = reinterpret_cast<uint64_t>(a.sink);
+  const uint64_t odir_b = reinterpret_cast<uint64_t>(a.odir);
 
   // PPT particles per thread and iteration (FP32: two chunks of kBlock, whose
   // loads are in flight together; one block scan ranks both), the next
@@ -513,7 +533,56 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     bool valid = act && !isnan(p[0]);
 
       // ---- velocity: trilinear gather of nodal E, Boris rotation with t = b
-      if constexpr ((MODE & M_VEL) != 0) {
+      if constexpr ((MODE & M_VEL) != 0 && sizeof(R) == 4) {
+        // FP32 particles: the same gather and rotation with fused multiply-adds
+        // and factored weights (FP32 makes no claim of bit equality with the
+        // reference, so the operation order is free)
+        if (valid && !from_in) {
+          const float fx = fminf(floorf(p[0]), (float)(a.N[0] - 1));
+          const float fy = fminf(floorf(p[1]), (float)(a.N[1] - 1));
+          const float fz = fminf(floorf(p[2]), (float)(a.N[2] - 1));
+          const int ci = (int)fx, cj = (int)fy, ck = (int)fz;
+          const float dx1 = p[0] - fx, dy1 = p[1] - fy, dz1 = p[2] - fz;
+          const float dx2 = 1.0f - dx1, dy2 = 1.0f - dy1, dz2 = 1.0f - dz1;
+          const float qm = (float)a.qm;
+          const float qz1 = qm * dz1, qz2 = qm * dz2;
+          const float a22 = dx2 * dy2, a12 = dx1 * dy2, a21 = dx2 * dy1, a11 = dx1 * dy1;
+          const float v1 = a22 * qz2, v2 = a12 * qz2, v3 = a21 * qz2, v4 = a11 * qz2;
+          const float v5 = a22 * qz1, v6 = a12 * qz1, v7 = a21 * qz1, v8 = a11 * qz1;
+          const int li = ci - o0, lj = cj - o1, lk = ck - o2;
+          float e[3];
+          if (li >= 0 && li < kTile && lj >= 0 && lj < kTile && lk >= 0 && lk < kTile) {
+            const int b0 = (li * kEW + lj) * kEW + lk;
+            const float4* E4 = reinterpret_cast<const float4*>(sE) + b0;
+            const float4 n1 = E4[0], n2 = E4[kEW * kEW], n3 = E4[kEW], n4 = E4[kEW * kEW + kEW];
+            const float4 n5 = E4[1], n6 = E4[kEW * kEW + 1], n7 = E4[kEW + 1], n8 = E4[kEW * kEW + kEW + 1];
+            e[0] = __builtin_fmaf(v8, n8.x, __builtin_fmaf(v7, n7.x, __builtin_fmaf(v6, n6.x, __builtin_fmaf(v5, n5.x,
+                   __builtin_fmaf(v4, n4.x, __builtin_fmaf(v3, n3.x, __builtin_fmaf(v2, n2.x, v1 * n1.x)))))));
+            e[1] = __builtin_fmaf(v8, n8.y, __builtin_fmaf(v7, n7.y, __builtin_fmaf(v6, n6.y, __builtin_fmaf(v5, n5.y,
+                   __builtin_fmaf(v4, n4.y, __builtin_fmaf(v3, n3.y, __builtin_fmaf(v2, n2.y, v1 * n1.y)))))));
+            e[2] = __builtin_fmaf(v8, n8.z, __builtin_fmaf(v7, n7.z, __builtin_fmaf(v6, n6.z, __builtin_fmaf(v5, n5.z,
+                   __builtin_fmaf(v4, n4.z, __builtin_fmaf(v3, n3.z, __builtin_fmaf(v2, n2.z, v1 * n1.z)))))));
+          } else {  // outside the tile (a bin left stale by an interrupted step): global gather
+            const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
+            const size_t g = (size_t)(ci + 1) * sI + (size_t)(cj + 1) * sJ + (ck + 1);
+            const double* Es[3] = {a.ex, a.ey, a.ez};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const double* E = Es[c] + g;
+              e[c] = v1 * (float)E[0] + v2 * (float)E[sI] + v3 * (float)E[sJ] + v4 * (float)E[sI + sJ] +
+                     v5 * (float)E[1] + v6 * (float)E[sI + 1] + v7 * (float)E[sJ + 1] + v8 * (float)E[sI + sJ + 1];
+            }
+          }
+          const float bx = (float)a.bx, by = (float)a.by, bz = (float)a.bz, boris = (float)a.boris;
+          const float vx1 = p[3] + e[0], vy1 = p[4] + e[1], vz1 = p[5] + e[2];
+          const float vxt = __builtin_fmaf(vy1, bz, __builtin_fmaf(-vz1, by, vx1));
+          const float vyt = __builtin_fmaf(vz1, bx, __builtin_fmaf(-vx1, bz, vy1));
+          const float vzt = __builtin_fmaf(vx1, by, __builtin_fmaf(-vy1, bx, vz1));
+          p[3] = vx1 + __builtin_fmaf(boris, __builtin_fmaf(vyt, bz, -vzt * by), e[0]);
+          p[4] = vy1 + __builtin_fmaf(boris, __builtin_fmaf(vzt, bx, -vxt * bz), e[1]);
+          p[5] = vz1 + __builtin_fmaf(boris, __builtin_fmaf(vxt, by, -vyt * bx), e[2]);
+        }
+      } else if constexpr ((MODE & M_VEL) != 0) {
         if (valid && !from_in) {
           const R x = p[0], y = p[1], z = p[2];
           int ci = (int)floor(x), cj = (int)floor(y), ck = (int)floor(z);
@@ -637,7 +706,25 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           const R w[8] = {dx2 * dy2 * dz2, dx1 * dy2 * dz2, dx2 * dy1 * dz2, dx1 * dy1 * dz2,
                           dx2 * dy2 * dz1, dx1 * dy2 * dz1, dx2 * dy1 * dz1, dx1 * dy1 * dz1};
           const int wi = di - o0 + 1, wj = dj - o1 + 1, wk = dk - o2 + 1;
-          if (fix_ok && wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW) {
+          const bool in_win = wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW;
+          if (sizeof(R) == 4 && in_win) {
+            // FP32 particles: kFix32 units per weight, the factored weights
+            // scaled and rounded in one multiply-add each (+0.5, then the
+            // conversion truncates), 32-bit values into the 64-bit window
+            const float s1 = (float)dz1 * kFix32, s2 = (float)dz2 * kFix32;
+            const float a22 = (float)(dx2 * dy2), a12 = (float)(dx1 * dy2), a21 = (float)(dx2 * dy1),
+                        a11 = (float)(dx1 * dy1);
+            unsigned long long* W = sR + (wi * kRW + wj) * kRW + wk;
+            auto fx = [](float v) -> unsigned long long { return (unsigned long long)(unsigned)v; };
+            atomicAdd(W, fx(__builtin_fmaf(a22, s2, 0.5f)));
+            atomicAdd(W + kRW * kRW, fx(__builtin_fmaf(a12, s2, 0.5f)));
+            atomicAdd(W + kRW, fx(__builtin_fmaf(a21, s2, 0.5f)));
+            atomicAdd(W + kRW * kRW + kRW, fx(__builtin_fmaf(a11, s2, 0.5f)));
+            atomicAdd(W + 1, fx(__builtin_fmaf(a22, s1, 0.5f)));
+            atomicAdd(W + kRW * kRW + 1, fx(__builtin_fmaf(a12, s1, 0.5f)));
+            atomicAdd(W + kRW + 1, fx(__builtin_fmaf(a21, s1, 0.5f)));
+            atomicAdd(W + kRW * kRW + kRW + 1, fx(__builtin_fmaf(a11, s1, 0.5f)));
+          } else if (sizeof(R) == 8 && fix_ok && in_win) {
             // 64-bit fixed point (kFix units per weight 1): integer sums do not
             // depend on the order the atomics land in, so the window, and rho,
             // are the same bit for bit from run to run
@@ -788,16 +875,19 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
             atomicAdd(&a.status[1], 1);
           }
         }
+        // one base address and one component stride per destination: each
+        // particle set is one allocation with its components a fixed stride
+        // apart, and the sink takes all six components in one slot
+        const uint64_t db = kind == 0 ? dst_b : (kind == 1 ? out_b : (kind == 2 ? str_b : snk_b));
+        const uint64_t ds = kind == 0 ? dst_s : (kind == 1 ? out_s : (kind == 2 ? str_s : 0ull));
+        uint64_t ad = db + (uint64_t)at * sizeof(R);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-          __attribute__((address_space(1))) R* d = kind == 0 ? sgpr_ptr(a.dst.c[c])
-                           : (kind == 1 ? sgpr_ptr(a.out.c[c]) : (kind == 2 ? sgpr_ptr(a.stray.c[c]) : sgpr_ptr(a.sink)));
-          d[at] = p[u][c];
+          *reinterpret_cast<__attribute__((address_space(1))) R*>(ad) = p[u][c];
+          ad += ds;
         }
-        __attribute__((address_space(1))) uint8_t* od =
-            kind == 1 ? sgpr_ptr(a.odir) + at
-                      : reinterpret_cast<__attribute__((address_space(1))) uint8_t*>(sgpr_ptr(a.sink)) + sink_at;
-        *od = (uint8_t)dir[u];
+        const uint64_t od = kind == 1 ? odir_b + (uint64_t)at : snk_b + (uint64_t)sink_at;
+        *reinterpret_cast<__attribute__((address_space(1))) uint8_t*>(od) = (uint8_t)dir[u];
         wb += ts[u];
         ob0 += tm[u];
       }
@@ -829,7 +919,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   }
   if constexpr (SM::kR) {
     double* P = a.rho_part + (size_t)t * (kRW * kRW * kRW);
-    const double scale = a.q / kFix;  // exact: kFix is a power of two
+    const double scale = a.q / (sizeof(R) == 4 ? (double)kFix32 : kFix);  // exact: powers of two
     for (int q = threadIdx.x; q < kRW * kRW * kRW; q += kBlock) P[q] = (double)(long long)sR[q] * scale;
   }
   if constexpr (SM::kD) {
@@ -1053,16 +1143,24 @@ __global__ void k_cell_keys(SoaPtr<R> f, int64_t n, int ny1, int nz1, int32_t* k
 }
 
 // sum of v^2 over n particles into per-block partials
+// Sum of vx^2 + vy^2 + vz^2 (FP64) over the live particles of the tile
+// regions, read in place: a fixed grid of workgroups strides over the tiles
+// and each writes one partial. A particle that a standalone position update
+// absorbed keeps its slot with x = NaN until the next re-bin;
+// Grid::getParticleEnergy skips it (GridParticleTreatment.cpp:104-118).
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_v2(SoaPtr<R> f, int64_t n, double* part) {
+__global__ __launch_bounds__(kBlock) void k_v2_tiles(SoaPtr<R> f, const int64_t* start, const int32_t* cnt,
+                                                     int ntiles, double* part) {
   __shared__ double s[kBlock];
   double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    // a particle that left the domain in a standalone position update keeps its
-    // slot with x = NaN until the next re-bin; Grid::getParticleEnergy skips it
-    if (isnan(f.c[0][i])) continue;
-    const double vx = f.c[3][i], vy = f.c[4][i], vz = f.c[5][i];
-    acc += vx * vx + vy * vy + vz * vz;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t b0 = start[t];
+    const int n = cnt[t];
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+      if (isnan(f.c[0][b0 + i])) continue;
+      const double vx = f.c[3][b0 + i], vy = f.c[4][b0 + i], vz = f.c[5][b0 + i];
+      acc += vx * vx + vy * vy + vz * vz;
+    }
   }
   s[threadIdx.x] = acc;
   __syncthreads();
@@ -1442,6 +1540,7 @@ TileArgs<R> tile_args(Ctx& c, int s) {
     a.obj[o].node_cmat = c.obj[o].node_cmat.p;
     a.obj[o].tmp = c.obj[o].tmp_charge.p ? c.obj[o].tmp_charge.p + (size_t)s * c.obj[o].ncmat : nullptr;
     a.obj[o].absorbed = c.obj[o].absorbed.p ? c.obj[o].absorbed.p + s : nullptr;
+    a.obj[o].lost = c.obj[o].absorbed.p ? c.obj[o].absorbed.p + c.S + s : nullptr;
   }
   a.cap_rec = c.cap_rec.p + (size_t)8 * c.cap_cap * s;
   a.cap_cap = c.cap_cap;
@@ -1624,10 +1723,26 @@ void launch_tiles(Ctx& c, int s) {
 
 // The binning pass of species s into its second region buffer (allocated on
 // first use, the size of the first); the buffers swap once the arrivals are in.
+// Whether species s can bin into a second region buffer: it is allocated on
+// first use (the size of the first) when the device has room for it twice
+// over (a later full re-bin gathers a flat copy of the species); otherwise
+// the species compacts in place.
+template <class R>
+bool reg2_ready(Ctx& c, int s) {
+  auto& b = bins<R>(c)[s];
+  if (b.reg2.blk.p && b.reg2.size() == b.reg.size()) return true;
+  b.reg2.release();
+  size_t fr = 0, tot = 0;
+  FOPIS_HIP(hipMemGetInfo(&fr, &tot));
+  const size_t need = 6 * b.reg.size() * sizeof(R);
+  if (fr < 2 * need + ((size_t)4 << 30)) return false;
+  b.reg2.alloc(b.reg.size());
+  return true;
+}
+
 template <class R, int MODE>
 void launch_tiles_db(Ctx& c, int s) {
   auto& b = bins<R>(c)[s];
-  if (b.reg2.size() != b.reg.size()) b.reg2.alloc(b.reg.size());
   TileArgs<R> a = tile_args<R>(c, s);
   a.dst = ptrs(b.reg2);
   k_tile<R, MODE, true><<<c.ntiles, kBlock, 0, c.stream>>>(a);
@@ -1656,6 +1771,7 @@ void after_bin(Ctx& c) {
       oa[o].node_cmat = c.obj[o].node_cmat.p;
       oa[o].tmp = c.obj[o].tmp_charge.p + (size_t)s * c.obj[o].ncmat;
       oa[o].absorbed = c.obj[o].absorbed.p + s;
+      oa[o].lost = c.obj[o].absorbed.p + c.S + s;
     }
     const fopis_species_desc& sd = c.d.species[s];
     k_capture<<<div_up(n, 128), 128, 0, c.stream>>>(c.cap_rec.p + (size_t)8 * c.cap_cap * s, n, c.cx, c.cy, c.cz,
@@ -1676,12 +1792,14 @@ void deposit_bin(Ctx& c, bool push) {
   FOPIS_HIP(hipMemsetAsync(c.density.p, 0, c.density.n * sizeof(double), st));
   FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), st));
   for (int o = 0; o < c.nobj; ++o)
-    if (c.obj[o].absorbed.p) FOPIS_HIP(hipMemsetAsync(c.obj[o].absorbed.p, 0, c.S * sizeof(unsigned long long), st));
-  const bool db = c.tile_double_buffer;
+    if (c.obj[o].absorbed.p)
+      FOPIS_HIP(hipMemsetAsync(c.obj[o].absorbed.p, 0, 2 * c.S * sizeof(unsigned long long), st));
+  std::vector<char> dbs(c.S, 0);
+  for (int s = 0; s < c.S; ++s) dbs[s] = c.tile_double_buffer && reg2_ready<R>(c, s);
   {
     Phase ph(c, "tile");
     for (int s = 0; s < c.S; ++s) {
-      if (db) {
+      if (dbs[s]) {
         if (push)
           launch_tiles_db<R, M_VEL | M_POS | M_DEP | M_BIN>(c, s);
         else
@@ -1699,10 +1817,9 @@ void deposit_bin(Ctx& c, bool push) {
     for (int s = 0; s < c.S; ++s) {
       auto& b = bins<R>(c)[s];
       TileArgs<R> a = tile_args<R>(c, s);
-      if (db) a.dst = ptrs(b.reg2);
+      if (dbs[s]) a.dst = ptrs(b.reg2);
       k_arrive<R><<<c.ntiles, kBlock, 0, st>>>(a);
-      if (db)
-        for (int q = 0; q < 6; ++q) b.reg.c[q].swap(b.reg2.c[q]);
+      if (dbs[s]) b.reg.swap(b.reg2);
     }
     FOPIS_HIP(hipGetLastError());
   }
@@ -1944,7 +2061,7 @@ static void grow_dev_inbox(Ctx& c, SpeciesBins<R>& b, int64_t n) {
     FOPIS_HIP(hipMemcpyAsync(bigger.c[q].p, b.dev_inbox.c[q].p, b.dev_n * sizeof(double), hipMemcpyDeviceToDevice,
                              c.stream));
   FOPIS_HIP(hipStreamSynchronize(c.stream));
-  for (int q = 0; q < 6; ++q) b.dev_inbox.c[q].swap(bigger.c[q]);
+  b.dev_inbox.swap(bigger);
 }
 
 template <class R>
@@ -2133,28 +2250,52 @@ void particles_cell_keys(Ctx& c, int s, int64_t cap, int32_t* keys, int64_t* n) 
     keys_impl<double>(c, s, cap, keys, n);
 }
 
+// Queues, for every species s, sum v^2 over its live particles into
+// sums[s]: part holds kEnergyBlocks partials per species (fixed-order sums,
+// so the result is the same from run to run).
 template <class R>
-static double energy_impl(Ctx& c) {
-  double res = 0.0;
-  c.red.alloc(std::max<size_t>(c.red.n, 1025));
+static void queue_v2(Ctx& c, double* part, double* sums) {
   for (int s = 0; s < c.S; ++s) {
-    Soa<R> flat;
-    const int64_t n = gather_all<R>(c, s, flat, false);
-    double e = 0.0;
-    if (n > 0) {
-      const int nb = 1024;
-      k_v2<R><<<nb, kBlock, 0, c.stream>>>(ptrs(flat), n, c.red.p);
-      k_sum_serial<<<1, 1, 0, c.stream>>>(c.red.p, nb, c.red.p + 1024);
-      FOPIS_HIP(hipMemcpyAsync(&e, c.red.p + 1024, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-      FOPIS_HIP(hipStreamSynchronize(c.stream));
+    auto& b = bins<R>(c)[s];
+    double* ps = part + (size_t)s * kEnergyBlocks;
+    if (b.cnt.p && c.ntiles > 0) {
+      k_v2_tiles<R><<<kEnergyBlocks, kBlock, 0, c.stream>>>(ptrs(b.reg), b.start.p, b.cnt.p, c.ntiles, ps);
+      k_sum_serial<<<1, 1, 0, c.stream>>>(ps, kEnergyBlocks, sums + s);
+    } else {
+      k_sum_serial<<<1, 1, 0, c.stream>>>(ps, 0, sums + s);
     }
+  }
+  FOPIS_HIP(hipGetLastError());
+}
+
+// 1/2 sum_s m_s size_s sum v^2 from the per-species sums copied to the host.
+static double kinetic(const Ctx& c, const double* v2) {
+  double res = 0.0;
+  for (int s = 0; s < c.S; ++s) {
     const fopis_species_desc& sd = c.d.species[s];
-    res += 0.5 * sd.mass * e * sd.size;
+    res += 0.5 * sd.mass * v2[s] * sd.size;
   }
   return res;
 }
 
-double particles_energy(Ctx& c) { return c.fp32 ? energy_impl<float>(c) : energy_impl<double>(c); }
+void particles_energy_queue(Ctx& c, double* part, double* sums) {
+  if (c.fp32)
+    queue_v2<float>(c, part, sums);
+  else
+    queue_v2<double>(c, part, sums);
+}
+
+double particles_energy_from(const Ctx& c, const double* v2) { return kinetic(c, v2); }
+
+double particles_energy(Ctx& c) {
+  const size_t np = (size_t)c.S * kEnergyBlocks;
+  c.red.alloc(std::max<size_t>(c.red.n, np + c.S));
+  particles_energy_queue(c, c.red.p, c.red.p + np);
+  std::vector<double> v2(c.S);
+  FOPIS_HIP(hipMemcpyAsync(v2.data(), c.red.p + np, c.S * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  return kinetic(c, v2.data());
+}
 
 template <class R>
 static void moments_impl(Ctx& c, int s, double* out) {

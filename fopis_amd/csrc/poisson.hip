@@ -203,6 +203,9 @@ __global__ __launch_bounds__(256) void k_norm_final(const double* part, int nb, 
 
 __global__ void k_reopen(int* st) { st[0] = 0; }
 
+// The relative residual the solve starts from, kept beside the running norms.
+__global__ void k_keep_start(double* dst) { dst[3] = dst[2]; }
+
 // Kernels over a level's interior give each thread the same column position
 // in kRows consecutive rows (b), issue every load of those rows before the
 // first store, and so keep several independent loads in flight per thread.
@@ -851,10 +854,11 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   k_solve_begin<<<1, 1, 0, c.stream>>>(st);
   k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, part, done, r64);
   k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, 0, 0);
+  k_keep_start<<<1, 1, 0, c.stream>>>(dst);
   // queue as many cycles as the previous solves needed, then two at a time
   int batch = std::max(1, std::min(c.mg_hist[0], c.mg_hist[1]));
   int32_t hs[2];
-  double hd[3];
+  double hd[4];
   // One cycle: the V-cycle, the FP64 update and the norm check. Its ~30
   // launches are captured once into a graph of kernel nodes (every pointer
   // and size in them is fixed for the context; the tolerance and the cycle
@@ -929,6 +933,7 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
     }
   FOPIS_HIP(hipGetLastError());
   *relres_out = hd[2];
+  c.relres0_last = hd[3];
   return cycles;
 }
 
@@ -983,6 +988,8 @@ void poisson_init(Ctx& c) {
   c.mg_part.alloc(2 * kResidBlocks + 4);
   const char* g = std::getenv("FOPIS_MG_GRAPH");
   c.mg_use_graph = !(g && g[0] == '0');
+  const char* cy = std::getenv("FOPIS_CARRY");
+  c.carry = !(cy && cy[0] == '0');
   int dev = 0;
   FOPIS_HIP(hipGetDevice(&dev));
   hipDeviceProp_t pr;

@@ -321,8 +321,11 @@ class Context:
         cur = np.zeros(self.S)
         cmap = np.zeros(self.S * max(n, 1))
         self._ok(self.L.fopis_get_object_state(self.h, o, C.byref(pot), C.byref(q), _dp(cur), _dp(cmap)))
+        lost = np.zeros(self.S, np.int64)
+        self.L.fopis_object_unattributed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
+        self._ok(self.L.fopis_object_unattributed(self.h, o, lost.ctypes.data_as(C.POINTER(C.c_int64))))
         return {"potential": pot.value, "total_charge": q.value, "current": cur,
-                "charge_map": cmap[: self.S * n].reshape(self.S, n)}
+                "charge_map": cmap[: self.S * n].reshape(self.S, n), "unattributed": lost}
 
     def set_object_charge_map(self, o: int, cmap: np.ndarray):
         a = np.ascontiguousarray(cmap, dtype=np.float64).ravel()
@@ -354,5 +357,9 @@ def solver_stats(ctx: "Context") -> dict:
     ctx._ok(ctx.L.fopis_solver_stats(ctx.h, C.byref(lc), C.byref(lr), C.byref(tc), C.byref(ts)))
     ctx.L.fopis_first_solve_cycles.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
     ctx._ok(ctx.L.fopis_first_solve_cycles(ctx.h, C.byref(fc)))
+    ctx.L.fopis_solver_start_residuals.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    r0 = (C.c_double * 3)()
+    ctx._ok(ctx.L.fopis_solver_start_residuals(ctx.h, r0))
     return {"last_cycles": lc.value, "last_relres": lr.value, "total_cycles": tc.value, "total_solves": ts.value,
-            "first_cycles": fc.value}
+            "first_cycles": fc.value, "start_relres_last": r0[0], "start_relres_first": r0[1],
+            "start_relres_second": r0[2]}

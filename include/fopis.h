@@ -228,6 +228,12 @@ int fopis_add_object_charge(fopis_ctx* ctx, int obj, const double* dq, const dou
  * charge_map (S*ncmat, may be NULL). */
 int fopis_get_object_state(fopis_ctx* ctx, int obj, double* potential, double* total_charge, double* current,
                            double* charge_map);
+/* Superparticles of each species (out[S]) that the last deposit captured in
+ * object obj without finding the face they entered by: their charge counts in
+ * the current but on no surface node, as in Spacecraft::
+ * distributeInnerParticleCharge, which prints "Surface cannot detect" for
+ * them (Spacecraft.cpp:573-589). */
+int fopis_object_unattributed(fopis_ctx* ctx, int obj, int64_t* out);
 /* Sets charge_map (S*ncmat), as initializeChargeMapOffset does (Spacecraft.cpp:160-183). */
 int fopis_set_object_charge_map(fopis_ctx* ctx, int obj, const double* charge_map);
 
@@ -249,6 +255,14 @@ int fopis_reset_timers(fopis_ctx* ctx);
  * RootGridField.cpp:86-88). Any pointer may be NULL. */
 int fopis_solver_stats(fopis_ctx* ctx, int32_t* last_cycles, double* last_relres, int64_t* total_cycles,
                        int64_t* total_solves);
+/* Relative residuals ||b - A phi||_2 / ||b||_2 the solves started from:
+ * out[0] the last solve's, out[1] the mean over the first solve of each
+ * main_loop_es step since fopis_reset_timers, out[2] the mean over the
+ * steps' second solves (after the conductor charge; 0 without conductors).
+ * The warm-start quality behind the cycle counts; the reference's SOR starts
+ * from the previous potential and reports no such figure
+ * (RootGridField.cpp:41-103). */
+int fopis_solver_start_residuals(fopis_ctx* ctx, double* out);
 /* Phase-space moments of species s in FP64 over its live particles:
  * out[0] count, out[1..3] mean x, y, z, out[4..6] mean vx, vy, vz,
  * out[7..9] variance of vx, vy, vz (the particle side of the FP32-vs-FP64

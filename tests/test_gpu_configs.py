@@ -16,7 +16,10 @@ step against invariants that hold whatever the particle order or the seed:
   total charge, which the capacitance redistribution puts back on the
   surface nodes (Spacecraft.cpp:708-758);
 * the conductor's charge changes by the step's current (captured minus
-  emitted superparticle charge, Spacecraft.cpp:615-626);
+  emitted superparticle charge, Spacecraft.cpp:615-626), less the charge of
+  captured particles whose entry face was not found (the reference prints
+  "Surface cannot detect" for them and keeps them out of the surface charge,
+  Spacecraft.cpp:573-589; a handful per million captures at most);
 * every Poisson solve reaches ||b - A phi||_2 / ||b||_2 <= 1e-8.
 
 Plus the device loader's law at the 256^3 size, and the learned-guess start
@@ -72,7 +75,11 @@ def _check_step(h, before, after, fp32):
     for ob0, ob1 in zip(before["objects"], after["objects"]):
         dq = ob1["total_charge"] - ob0["total_charge"]
         cur = float(np.sum(ob1["current"]))
-        assert abs(dq - cur) <= 1e-9 * max(abs(cur), abs(ob1["total_charge"]), 1.0), (dq, cur)
+        lost = ob1["unattributed"]
+        q_lost = sum(float(lost[s]) * h.desc.species[s].charge * h.desc.species[s].size for s in range(h.S))
+        assert abs(dq - (cur - q_lost)) <= 1e-9 * max(abs(cur), abs(ob1["total_charge"]), 1.0), (dq, cur, lost)
+        captured = sum(after["ledger"][s]["captured"] - before["ledger"][s]["captured"] for s in range(h.S))
+        assert int(lost.sum()) <= max(2, captured // 1000), (lost, captured)
 
 
 def _run(name, steps, fp32):
