@@ -377,6 +377,14 @@ int fopis_solver_start_residuals(fopis_ctx* ctx, double* out) {
   });
 }
 
+int fopis_energy_noalloc(fopis_ctx* ctx, double* out) {
+  NEED_CTX
+  return guarded([&] {
+    if (!out) throw Error(FOPIS_E_ARG, "null output");
+    energy_noalloc(ctx->c, out);
+  });
+}
+
 int fopis_set_object(fopis_ctx* ctx, int obj, const uint8_t* cell_mask, int32_t ncmat, const int32_t* cmat_ijk,
                      int32_t fixed, double fixed_potential) {
   NEED_CTX

@@ -247,6 +247,7 @@ struct MgLevel {
 struct GuessNet;  // learned initial guess (guess.hip)
 
 struct Ctx {
+  double* noalloc = nullptr;  // this device's copy of the allocation-free entry points' scratch array
   fopis_desc d{};
   // the particle kernels' sort and binning scratch (owned here, so it goes with the context)
   std::shared_ptr<void> particle_scratch;
@@ -341,6 +342,10 @@ constexpr int kEnergyBlocks = 1024;
 void particles_energy_queue(Ctx& c, double* part, double* sums);
 // 1/2 sum_s m_s size_s v2[s].
 double particles_energy_from(const Ctx& c, const double* v2);
+// Device address of this device's scratch array (set up once per device, with it current).
+void noalloc_setup(Ctx& c);
+// fopis_energy computed with the partials in that array (allocates nothing).
+void energy_noalloc(Ctx& c, double* out);
 void particles_moments(Ctx& c, int s, double* out);
 void particles_ledger(Ctx& c, int s, int64_t* out);
 

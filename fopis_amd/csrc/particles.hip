@@ -16,6 +16,7 @@
 // on identical inputs.
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "fopis_internal.hpp"
 #include "radix.hpp"
@@ -1851,6 +1852,7 @@ void particles_init(Ctx& c) {
   c.sink.alloc(kSink);  // doubles: kSink elements of either precision, and kSink bytes
   c.status.alloc(kStat * FOPIS_MAX_SPECIES);
   FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), c.stream));
+  noalloc_setup(c);
   for (int s = 0; s < c.S; ++s) {
     if (c.fp32)
       bin_flat<float>(c, s, c.sp32[s].reg, 0, false);
@@ -2328,6 +2330,56 @@ void particles_moments(Ctx& c, int s, double* out) {
     moments_impl<float>(c, s, out);
   else
     moments_impl<double>(c, s, out);
+}
+
+// ------------------------------------------- allocation-free energy reduction
+//
+// fopis_energy_noalloc keeps its partial sums in this file-scope array instead
+// of library-allocated memory: kEnergyBlocks partials per species and for the
+// field, then the S + 1 sums. Every call overwrites all of them (nothing has to
+// start at zero). Each device holds its own copy for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device, never through the array's name.
+constexpr size_t kNoAllocDoubles = (size_t)(FOPIS_MAX_SPECIES + 1) * kEnergyBlocks + FOPIS_MAX_SPECIES + 1;
+static __device__ __align__(256) double noalloc_scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+void noalloc_setup(Ctx& c) {
+  int dev = 0;
+  FOPIS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Error(FOPIS_E_DEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess || !p)
+      throw Error(FOPIS_E_DEVICE, std::string("scratch array address: ") + hipGetErrorString(e));
+    g_noalloc_ptr[dev] = static_cast<double*>(p);
+  }
+  c.noalloc = g_noalloc_ptr[dev];
+}
+
+void energy_noalloc(Ctx& c, double* out) {
+  if (!c.noalloc) throw Error(FOPIS_E_STATE, "scratch array not set up for this context");
+  const size_t need = (size_t)(c.S + 1) * kEnergyBlocks + c.S + 1;
+  if (c.S < 0 || c.S > FOPIS_MAX_SPECIES || need > kNoAllocDoubles)
+    throw Error(FOPIS_E_ARG, "energy partials of " + std::to_string(c.S) + " species exceed the scratch array (" +
+                                 std::to_string(FOPIS_MAX_SPECIES) + " species at most)");
+  double* part = c.noalloc;
+  double* fpart = part + (size_t)c.S * kEnergyBlocks;
+  double* sums = fpart + kEnergyBlocks;
+  particles_energy_queue(c, part, sums);
+  fields_energy_queue(c, fpart, sums + c.S);
+  double h[FOPIS_MAX_SPECIES + 1];
+  FOPIS_HIP(hipMemcpyAsync(h, sums, (size_t)(c.S + 1) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  out[0] = particles_energy_from(c, h);
+  out[1] = 0.5 * c.d.eps0 * h[c.S];
 }
 
 }  // namespace fopis

@@ -201,6 +201,16 @@ int fopis_set_field(fopis_ctx* ctx, int which, int s, const double* in);
  * Field::getEfieldEnergy (Field.cpp:194-213), FP64 reductions on the device.
  * out[0] = particle energy, out[1] = field energy. */
 int fopis_energy(fopis_ctx* ctx, double* out);
+/* fopis_energy without any device allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * partial sums live in a fixed array of the library, one per device, of
+ * (FOPIS_MAX_SPECIES + 1) * 1024 + FOPIS_MAX_SPECIES + 1 doubles (74 KB),
+ * held until the process ends; a context with more species than that is
+ * refused. The result equals fopis_energy's bit for bit (same kernels, same
+ * summation order). Every call on a device shares that device's array:
+ * calls that overlap on one device race. */
+int fopis_energy_noalloc(fopis_ctx* ctx, double* out);
 
 /* ---- conductors (Spacecraft) ------------------------------------------------- */
 
