@@ -156,7 +156,8 @@ def poisson_cpu_baseline(desc, cfg_name: str, phi_prev, rho_new, threads: int, g
 def learned_leg(h, steps: int, weights: str) -> dict:
     """The first Poisson solve of each step from the network's guess (guess
     inference + multigrid to the same tolerance) beside the same solve warm-
-    started from the previous step's phi, both over `steps` steps."""
+    started from the previous step's phi, both over `steps` steps, with the
+    relative residual each one starts from."""
     import numpy as np
 
     from fopis_amd import lib as F
@@ -182,6 +183,7 @@ def learned_leg(h, steps: int, weights: str) -> dict:
         st = F.solver_stats(ctx)
         out[key] = {"first_solve_ms": ctx.phase_time("poisson_first")[0],
                     "first_solve_cycles": st["first_cycles"] / steps,
+                    "start_relres": st["start_relres_first"],
                     "guess_ms": ctx.phase_time("guess")[0] if mode else 0.0}
     g.set_mode(0)
     out["speedup"] = out["plain"]["first_solve_ms"] / max(out["learned"]["first_solve_ms"], 1e-9)

@@ -66,6 +66,23 @@ static void check_desc(const fopis_desc& d) {
 
 // With the learned guess on, the first Poisson solve of a step starts from
 // the network's potential instead of the previous step's.
+// The first solve of a step and where it starts: the network's potential
+// (guess mode 1), or the warm start (in the fused step with conductors, the
+// previous potential less its conductor response), which guess mode 2
+// corrects by the network's potential for its residual and mode 3 records.
+static void first_solve(Ctx& c, bool carry, double* rr) {
+  Phase ph(c, "poisson_first");
+  const int mode = guess_mode(c);
+  if (mode == 1) {
+    guess_apply(c);
+  } else {
+    if (carry && c.nobj > 0) objects_carry_first(c);
+    if (mode >= 2) guess_before_solve(c);
+  }
+  poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, rr, nullptr);
+  if (mode == 3) guess_after_solve(c);
+}
+
 // In the fused step (carry), the two solves of a step with conductors start
 // from the warm starts objects_carry_* forms.
 static void update_rho(Ctx& c, bool push, bool carry = false) {
@@ -73,14 +90,7 @@ static void update_rho(Ctx& c, bool push, bool carry = false) {
   objects_sum_whole_charge(c);
   if (c.nobj > 0) {
     double rr = 0.0;
-    {
-      Phase ph(c, "poisson_first");  // guess (when on) + the step's first solve
-      if (guess_mode(c) == 1)
-        guess_apply(c);
-      else if (carry)
-        objects_carry_first(c);
-      poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
-    }
+    first_solve(c, carry, &rr);
     c.mg_first_cycles_total += c.mg_cycles_last;
     c.relres0_first += c.relres0_last;
     ++c.n_first;
@@ -321,11 +331,7 @@ int fopis_main_loop_es(fopis_ctx* ctx) {
     update_rho(c, true, true);
     double rr = 0.0;
     if (c.nobj == 0) {
-      {
-        Phase ph(c, "poisson_first");
-        if (guess_mode(c) == 1) guess_apply(c);
-        poisson_solve(c, c.d.poisson_tol, c.d.mg_max_cycles, &rr, nullptr);
-      }
+      first_solve(c, false, &rr);
       c.mg_first_cycles_total += c.mg_cycles_last;
       c.relres0_first += c.relres0_last;
       ++c.n_first;
@@ -578,6 +584,14 @@ int fopis_guess_set_params(fopis_ctx* ctx, const float* params, double out_scale
 int fopis_guess_set_mode(fopis_ctx* ctx, int mode) {
   NEED_CTX
   return guarded([&] { guess_set_mode(ctx->c, mode); });
+}
+
+int fopis_guess_recorded(fopis_ctx* ctx, int32_t* n) {
+  NEED_CTX
+  return guarded([&] {
+    if (!n) throw Error(FOPIS_E_ARG, "null output");
+    *n = guess_recorded(ctx->c);
+  });
 }
 
 int fopis_guess_set_loss(fopis_ctx* ctx, int kind) {

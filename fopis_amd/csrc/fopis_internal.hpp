@@ -392,6 +392,10 @@ void guess_set_mode(Ctx& c, int mode);
 void guess_set_loss(Ctx& c, int kind);
 int guess_mode(const Ctx& c);
 void guess_apply(Ctx& c);
+// Modes 2 (network correction of the warm start) and 3 (record warm-start pairs).
+void guess_before_solve(Ctx& c);
+void guess_after_solve(Ctx& c);
+int guess_recorded(const Ctx& c);
 void guess_dataset_alloc(Ctx& c, int capacity);
 double guess_dataset_store(Ctx& c, int slot);
 void guess_dataset_get(Ctx& c, int slot, uint16_t* rho_hat, uint8_t* mask, uint16_t* phi_hat);

@@ -76,6 +76,7 @@ struct ConvArgs {
   double* phi;           // EPI_PHI: glue-layout potential
   const double* scal;    // EPI_PHI: scal[0] = input scale sigma
   double phi_k;          // EPI_PHI: phi = out * sigma * phi_k
+  int phi_add;           // EPI_PHI: add to phi instead of replacing it
   int64_t gsx, gsy, goff;
   int vn0, vn1, vn2;     // EPI_PHI: voxels that map to interior nodes
   int n0, n1, n2;        // output grid
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs A) {
         } else if (EPI == EPI_F32) {
           A.y32[v] = val;
         } else if (i < A.vn0 && j < A.vn1 && k < A.vn2) {
-          A.phi[A.goff + i * A.gsx + j * A.gsy + k] = (double)val * A.scal[0] * A.phi_k;
+          double* p = A.phi + A.goff + i * A.gsx + j * A.gsy + k;
+          *p = (A.phi_add ? *p : 0.0) + (double)val * A.scal[0] * A.phi_k;
         }
       }
     }
@@ -511,6 +513,30 @@ __global__ void k_features_field(const double* rho, const uint8_t* cm, Geo G, co
   *reinterpret_cast<u32x4*>(x0 + v * 8) = o;
 }
 
+// eps0 (b - A phi) at the network's interior nodes, b = rho / eps0 and A the
+// 7-point operator (6 phi minus the neighbours, dx = 1): the charge whose
+// potential the warm start phi is still missing.
+__global__ void k_warm_resid(const double* phi, const double* rho, double eps0, Geo G, double* out) {
+  const int64_t n = (int64_t)G.vn0 * G.vn1 * G.vn2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q % G.vn2), j = (int)((q / G.vn2) % G.vn1), i = (int)(q / ((int64_t)G.vn1 * G.vn2));
+  const int64_t g = G.goff + i * G.gsx + j * G.gsy + k;
+  const double ap = 6.0 * phi[g] - (phi[g - G.gsx] + phi[g + G.gsx] + phi[g - G.gsy] + phi[g + G.gsy] + phi[g - 1] +
+                                    phi[g + 1]);
+  out[g] = rho[g] - eps0 * ap;
+}
+
+// d = phi - phi_start at the interior nodes (in place into phi_start).
+__global__ void k_phi_delta(const double* phi, double* d, Geo G) {
+  const int64_t n = (int64_t)G.vn0 * G.vn1 * G.vn2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int k = (int)(q % G.vn2), j = (int)((q / G.vn2) % G.vn1), i = (int)(q / ((int64_t)G.vn1 * G.vn2));
+  const int64_t g = G.goff + i * G.gsx + j * G.gsy + k;
+  d[g] = phi[g] - d[g];
+}
+
 // Loss 0.5/N * sum (out - target/k)^2 over voxels of interior nodes, N their count;
 // gradient into channel 0 of g [voxel][16].
 __global__ __launch_bounds__(256) void k_loss(const float* out, const uint16_t* target, float inv_k, Geo G,
@@ -674,6 +700,9 @@ struct GuessNet {
   DBuf<float> wpart, cpart;  // weight- and bias-gradient partial sums
   DBuf<double> dpart, scal, loss_acc;
   DBuf<uint8_t> cellmask;  // training-data objects
+  DBuf<double> rwarm;      // eps0 (b - A phi) of the warm start at the interior nodes (modes 2 and 3)
+  DBuf<double> phi_start;  // mode 3: the warm start, to form phi_solution - phi_start after the solve
+  int record_slot = 0;     // mode 3: the dataset slot the next pair goes to
   bool use_cellmask = false;
   DBuf<Box> boxes;
   // dataset of training pairs
@@ -843,7 +872,7 @@ void alloc_train(GuessNet& g) {
 }
 
 // Forward pass from g.X0. epi = EPI_F32 writes g.OUT, EPI_PHI writes phi.
-void forward(Ctx& c, GuessNet& g, int epi) {
+void forward(Ctx& c, GuessNet& g, int epi, bool phi_add = false) {
   hipStream_t st = c.stream;
   auto conv = [&](int li, const uint16_t* x, const uint16_t* add, uint16_t* y) {
     GuessLayer& l = g.L[li];
@@ -881,6 +910,7 @@ void forward(Ctx& c, GuessNet& g, int epi) {
     a.phi = c.phi.p;
     a.scal = g.scal.p;
     a.phi_k = g.k_out / c.d.eps0;
+    a.phi_add = phi_add ? 1 : 0;
     a.gsx = g.G.gsx;
     a.gsy = g.G.gsy;
     a.goff = g.G.goff;
@@ -1136,12 +1166,64 @@ void guess_set_loss(Ctx& c, int kind) {
 }
 
 void guess_set_mode(Ctx& c, int mode) {
-  if (mode != 0 && mode != 1) throw Error(FOPIS_E_ARG, "guess mode must be 0 or 1");
-  if (mode == 1) net(c);
-  if (c.guess) c.guess->mode = mode;
+  if (mode < 0 || mode > 3) throw Error(FOPIS_E_ARG, "guess mode must be 0, 1, 2 or 3");
+  if (mode != 0) net(c);
+  if (c.guess) {
+    c.guess->mode = mode;
+    if (mode == 3) c.guess->record_slot = 0;
+  }
 }
 
 int guess_mode(const Ctx& c) { return c.guess ? c.guess->mode : 0; }
+
+// Modes 2 and 3 before the first solve of a step: the warm start's residual
+// (in charge units) into rwarm. Mode 2 then adds the network's potential for
+// that charge to phi; mode 3 keeps phi for the pair recorded after the solve.
+void guess_before_solve(Ctx& c) {
+  GuessNet& g = net(c);
+  Phase ph(c, "guess");
+  g.rwarm.alloc(c.nnode);
+  const int64_t n = (int64_t)g.G.vn0 * g.G.vn1 * g.G.vn2;
+  k_warm_resid<<<div_up(n, 256), 256, 0, c.stream>>>(c.phi.p, c.rho.p, c.d.eps0, g.G, g.rwarm.p);
+  if (g.mode == 3) {
+    g.phi_start.alloc(c.nnode);
+    FOPIS_HIP(hipMemcpyAsync(g.phi_start.p, c.phi.p, c.nnode * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    return;
+  }
+  compute_scale(c, g, g.rwarm.p);
+  const uint8_t* cm = g.use_cellmask ? g.cellmask.p : (c.have_mask ? c.mask.p : nullptr);
+  k_features_field<<<div_up(nvox_of(g.lv[0]), 256), 256, 0, c.stream>>>(g.rwarm.p, cm, g.G, g.scal.p, g.X0.p);
+  forward(c, g, EPI_PHI, true);
+  FOPIS_HIP(hipGetLastError());
+}
+
+// Mode 3 after the solve: the pair (eps0 r0 / sigma, mask, (phi - phi_start) eps0 / sigma)
+// into the next dataset slot, while there is room.
+void guess_after_solve(Ctx& c) {
+  GuessNet& g = net(c);
+  if (g.mode != 3 || g.record_slot >= g.capacity || !g.phi_start.p) return;
+  const int64_t n = (int64_t)g.G.vn0 * g.G.vn1 * g.G.vn2;
+  k_phi_delta<<<div_up(n, 256), 256, 0, c.stream>>>(c.phi.p, g.phi_start.p, g.G);
+  compute_scale(c, g, g.rwarm.p);
+  const int slot = g.record_slot++;
+  const int64_t nv = nvox_of(g.lv[0]);
+  const uint8_t* cm = g.use_cellmask ? g.cellmask.p : (c.have_mask ? c.mask.p : nullptr);
+  k_store_pair<<<div_up(nv, 256), 256, 0, c.stream>>>(g.rwarm.p, g.phi_start.p, cm, g.G, g.scal.p, c.d.eps0,
+                                                     g.ds_rho.p + (size_t)slot * nv, g.ds_mask.p + (size_t)slot * nv,
+                                                     g.ds_phi.p + (size_t)slot * nv);
+  std::vector<uint16_t> h(nv);
+  FOPIS_HIP(hipMemcpyAsync(h.data(), g.ds_phi.p + (size_t)slot * nv, nv * sizeof(uint16_t), hipMemcpyDeviceToHost,
+                           c.stream));
+  FOPIS_HIP(hipStreamSynchronize(c.stream));
+  double s = 0.0;
+  for (int64_t v = 0; v < nv; ++v) {
+    const double x = bf2f(h[v]);
+    s += x * x;
+  }
+  g.ds_phi_rms[slot] = std::sqrt(s / ((double)g.G.vn0 * g.G.vn1 * g.G.vn2));
+}
+
+int guess_recorded(const Ctx& c) { return c.guess ? c.guess->record_slot : 0; }
 
 // phi at interior nodes <- network(rho[0], conductor mask).
 void guess_apply(Ctx& c) {

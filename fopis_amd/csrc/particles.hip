@@ -508,10 +508,14 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   // Every lane loads from the tile region (a lane past the region's particles
   // reads its first slot, which exists: a region holds at least 64), so each
   // iteration issues the same loads whatever its lanes hold.
+  // the tile region's first element of each component as a uniform address
+  // and a 32-bit element index: loads take the scalar base plus a vector offset
+  const R* const rb = a.reg.c[0] + base;
+  const int64_t rs = a.reg.c[1] - a.reg.c[0];
   auto fetch = [&](int idx, R* q) {
-    const int j = idx < nreg ? idx : 0;
+    const unsigned j = idx < nreg ? (unsigned)idx : 0u;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) q[c] = a.reg.c[c][base + j];
+    for (int c = 0; c < 6; ++c) q[c] = sgpr_ptr(rb + c * rs)[j];
     if (idx >= nreg) {
       if (idx < ntot) {
 #pragma unroll

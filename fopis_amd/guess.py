@@ -61,6 +61,7 @@ def _bind(L):
     L.fopis_guess_get_params.argtypes = [vp, fp, dp]
     L.fopis_guess_set_params.argtypes = [vp, fp, d]
     L.fopis_guess_set_mode.argtypes = [vp, C.c_int]
+    L.fopis_guess_recorded.argtypes = [vp, C.POINTER(i32)]
     L.fopis_guess_set_loss.argtypes = [vp, C.c_int]
     L.fopis_guess_apply.argtypes = [vp]
     L.fopis_guess_dataset_alloc.argtypes = [vp, i32]
@@ -116,6 +117,12 @@ class Guess:
 
     def set_mode(self, mode: int):
         _check(self.L, self.L.fopis_guess_set_mode(self.ctx.h, mode))
+
+    def recorded(self) -> int:
+        """Warm-start pairs recorded since mode 3 was set."""
+        n = C.c_int32()
+        _check(self.L, self.L.fopis_guess_recorded(self.ctx.h, C.byref(n)))
+        return n.value
 
     def set_loss(self, kind: int):
         """0: potential MSE; 1: the multigrid's starting residual ||A(k out) - rho_hat||^2."""

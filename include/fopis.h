@@ -318,8 +318,15 @@ int fopis_guess_num_params(fopis_ctx* ctx, int64_t* n);
 int fopis_guess_get_params(fopis_ctx* ctx, float* params, double* out_scale);
 int fopis_guess_set_params(fopis_ctx* ctx, const float* params, double out_scale);
 /* 0: every solve starts from the previous phi (reference behaviour);
- * 1: the first solve of each step starts from the network's guess. */
+ * 1: the first solve of each step starts from the network's guess for rho;
+ * 2: the first solve starts from the previous phi plus the network's guess
+ *    for the residual charge eps0 (b - A phi) of that warm start;
+ * 3: as 0, and the first solve of each step records the pair (that
+ *    residual, phi_solution - phi_start) into the next dataset slot
+ *    (fopis_guess_recorded counts them; the counter restarts at 0). */
 int fopis_guess_set_mode(fopis_ctx* ctx, int mode);
+/* Pairs recorded since mode 3 was set. */
+int fopis_guess_recorded(fopis_ctx* ctx, int32_t* n);
 /* Training objective: 0 = 0.5*mean((out - phi_hat/k)^2) (default);
  * 1 = 0.5*mean((A(k*out) - rho_hat)^2), A the 7-point operator: the relative
  * residual the multigrid starts from, which is what sets its cycle count. */
