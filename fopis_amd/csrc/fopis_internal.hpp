@@ -26,9 +26,9 @@ constexpr int kDW = kTile + 2;  // density window: cells of the tile +-1
 constexpr int kBlock = 256;
 constexpr int kSink = 1 << 16;  // elements of the tile kernel's store sink (a power of two)
 // Per-species status words of the particle kernels: [0] strays, [1] lost
-// (stray or capture buffer full), [2] deposits outside the tile window,
-// [3] captured by a conductor, [4] absorbed at a domain face, [5] staged
-// particles that joined the tile regions.
+// (stray buffer full), [2] deposits outside the tile window, [3] captured by
+// a conductor, [4] absorbed at a domain face, [5] staged particles that
+// joined the tile regions, [6] captured past the capture buffer.
 constexpr int kStat = 8;
 
 struct Error : std::runtime_error {

@@ -9,6 +9,7 @@
 //   Cmatrix::total             include/Spacecraft/Cmatrix.hpp:36-44
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "fopis_internal.hpp"
@@ -419,21 +420,24 @@ __global__ void k_pair_products(const double* S, int m, int w, double* P) {
   P[i] = S[(int64_t)k * w + a] * S[(int64_t)k * w + b];
 }
 
-// B(col,row): one wavefront per (col, row) pair, lanes over k_a
+// B(col,row): one wavefront per (col, row) pair, lanes over k_a; a bounded
+// grid strides over the pairs (n^2 wavefronts of 64 lanes would pass the
+// 2^32 work-items a launch can hold once n exceeds 8192)
 __global__ __launch_bounds__(256) void k_green_pairs(const double* __restrict__ U, const double* __restrict__ Sa,
                                                      const int32_t* __restrict__ box, int n, int ma, int wa,
                                                      int wb, int wc, double scale, double* __restrict__ B) {
-  const int64_t pair = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (pair >= (int64_t)n * n) return;
-  const int col = (int)(pair / n), row = (int)(pair % n);
-  const int aa = box[3 * col], ab = box[3 * col + 1], ac = box[3 * col + 2];
-  const int ba = box[3 * row], bb = box[3 * row + 1], bc = box[3 * row + 2];
-  const double* u = U + (((int64_t)ab * wb + bb) * (wc * wc) + (int64_t)ac * wc + bc) * ma;
-  double s = 0.0;
-  for (int k = lane; k < ma; k += 64) s += Sa[(int64_t)k * wa + aa] * Sa[(int64_t)k * wa + ba] * u[k];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) B[(size_t)col * n + row] = s * scale;
+  const int64_t npair = (int64_t)n * n;
+  for (int64_t pair = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); pair < npair; pair += (int64_t)gridDim.x * 4) {
+    const int col = (int)(pair / n), row = (int)(pair % n);
+    const int aa = box[3 * col], ab = box[3 * col + 1], ac = box[3 * col + 2];
+    const int ba = box[3 * row], bb = box[3 * row + 1], bc = box[3 * row + 2];
+    const double* u = U + (((int64_t)ab * wb + bb) * (wc * wc) + (int64_t)ac * wc + bc) * ma;
+    double s = 0.0;
+    for (int k = lane; k < ma; k += 64) s += Sa[(int64_t)k * wa + aa] * Sa[(int64_t)k * wa + ba] * u[k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) B[(size_t)col * n + row] = s * scale;
+  }
 }
 
 // Fills B (n x n, (col,row) at col*n+row) from the sine-transform Green's
@@ -457,6 +461,12 @@ static bool spectral_unit_responses(Ctx& c, ObjectDev& ob, double* B) {
   // ord = {a, b, c}: a largest width, c smallest
   std::sort(ord, ord + 3, [&](int x, int y) { return w[x] > w[y] || (w[x] == w[y] && x < y); });
   const int A_ = ord[0], B_ = ord[1], C_ = ord[2];
+  {
+    // beyond 16 GB of intermediate responses (surface nodes spread over a
+    // wide box) the caller's per-node solves are the cheaper route
+    const double nU = (double)w[C_] * w[C_] * m[A_];
+    if (nU > INT32_MAX || (double)w[B_] * w[B_] * nU * sizeof(double) > 16.0 * (1 << 30)) return false;
+  }
   hipStream_t st = c.stream;
   // sine tables S_d[k][a] (node lo_d + a) and eigenvalues, in FP64 on the host
   DBuf<double> S[3], L[3];
@@ -471,10 +481,13 @@ static bool spectral_unit_responses(Ctx& c, ObjectDev& ob, double* B) {
     }
     S[d].alloc(hs.size());
     L[d].alloc(hl.size());
-    FOPIS_HIP(hipMemcpyAsync(S[d].p, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    FOPIS_HIP(hipMemcpyAsync(L[d].p, hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    // synchronous: the host tables go out of scope with this iteration
+    FOPIS_HIP(hipStreamSynchronize(st));
+    FOPIS_HIP(hipMemcpy(S[d].p, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice));
+    FOPIS_HIP(hipMemcpy(L[d].p, hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   const int ma = m[A_], mb = m[B_], mc = m[C_], wa = w[A_], wb = w[B_], wc = w[C_];
+  const int64_t nU = (int64_t)wc * wc * ma;
   DBuf<double> invl, Pc, Pb, T, U;
   invl.alloc((size_t)mb * mc * ma);
   k_inv_lambda<<<div_up((int64_t)ma * mb * mc, 256), 256, 0, st>>>(L[A_].p, L[B_].p, L[C_].p, ma, mb, mc, invl.p);
@@ -488,8 +501,6 @@ static bool spectral_unit_responses(Ctx& c, ObjectDev& ob, double* B) {
       Pc.p, invl.p, T.p, wc * wc, ma, mc, (int64_t)mc * ma, (int64_t)wc * wc * ma);
   invl.release();
   // U (wb^2 x wc^2 ma) = Pb (wb^2 x mb) x T (mb x wc^2 ma)
-  const int64_t nU = (int64_t)wc * wc * ma;
-  if (nU > INT32_MAX) return false;
   U.alloc((size_t)wb * wb * nU);
   k_dgemm<<<dim3(div_up(nU, kGT), div_up(wb * wb, kGT), 1), 256, 0, st>>>(Pb.p, T.p, U.p, wb * wb, (int)nU, mb, 0,
                                                                           0);
@@ -505,7 +516,8 @@ static bool spectral_unit_responses(Ctx& c, ObjectDev& ob, double* B) {
   box.alloc(hbox.size());
   FOPIS_HIP(hipMemcpyAsync(box.p, hbox.data(), hbox.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
   const int64_t pairs = (int64_t)n * n;
-  k_green_pairs<<<div_up(pairs, 4), 256, 0, st>>>(U.p, S[A_].p, box.p, n, ma, wa, wb, wc, 1.0 / c.d.eps0, B);
+  const int64_t gblocks = std::min<int64_t>(div_up(pairs, 4), (int64_t)1 << 20);
+  k_green_pairs<<<(unsigned)gblocks, 256, 0, st>>>(U.p, S[A_].p, box.p, n, ma, wa, wb, wc, 1.0 / c.d.eps0, B);
   FOPIS_HIP(hipGetLastError());
   FOPIS_HIP(hipStreamSynchronize(st));
   return true;
@@ -519,7 +531,9 @@ void objects_build_cmatrix(Ctx& c, int o) {
   const int64_t nn = (int64_t)c.nnode;
   DBuf<double> B;
   B.alloc((size_t)n * n);
-  if (!spectral_unit_responses(c, ob, B.p)) {
+  const char* sp = std::getenv("FOPIS_CMAT_SPECTRAL");
+  const bool spectral = !(sp && sp[0] == '0');  // FOPIS_CMAT_SPECTRAL=0: one solve per node
+  if (!spectral || !spectral_unit_responses(c, ob, B.p)) {
     // one unit-charge multigrid solve per surface node (RootGridObject.cpp:102-125)
     std::vector<int64_t> nodes(n);
     FOPIS_HIP(hipMemcpy(nodes.data(), ob.cmat_node.p, n * sizeof(int64_t), hipMemcpyDeviceToHost));

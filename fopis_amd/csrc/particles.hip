@@ -697,7 +697,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
               for (int c = 0; c < 6; ++c) r[c] = (double)p[c];
               r[6] = (double)(m - 1);
             } else {
-              atomicAdd(&a.status[1], 1);
+              atomicAdd(&a.status[6], 1);
             }
             valid = false;
           }
@@ -1761,7 +1761,13 @@ void after_bin(Ctx& c) {
   FOPIS_HIP(hipStreamSynchronize(c.stream));
   for (auto& b : bins<R>(c)) b.upload_pending = false;
   for (int s = 0; s < c.S; ++s) {
-    if (stat[kStat * s + 1] > 0) throw Error(FOPIS_E_NOMEM, "stray or capture buffer overflow: particles were lost");
+    if (stat[kStat * s + 1] > 0)
+      throw Error(FOPIS_E_NOMEM, "stray buffer overflow (" + std::to_string(stat[kStat * s + 1]) +
+                                     " particles past its capacity): particles were lost");
+    if (stat[kStat * s + 6] > 0)
+      throw Error(FOPIS_E_NOMEM, "capture buffer overflow (" + std::to_string(stat[kStat * s + 3]) +
+                                     " captured in one deposit, room for " + std::to_string(c.cap_cap) +
+                                     "): particles were lost");
     auto& b = bins<R>(c)[s];
     b.captured += stat[kStat * s + 3];
     b.absorbed += stat[kStat * s + 4];

@@ -42,8 +42,28 @@ def _ctx_with_object(g, Cm=None):
     return ctx
 
 
-def test_cmatrix_device_build_is_inverse_of_exact_b():
-    g = _oracle(cmatrix=False)
+def _union_obj(tmp_path, nx):
+    """Two overlapping boxes of unequal widths per axis (cells [lo, hi)), so
+    the sine-transform build orders its axes by width (z, y, x here)."""
+    from fopis_amd.voxel_obj import voxels_from_boxes, write_obj
+
+    boxes = [((9, 7, 5), (13, 13, 17)), ((10, 11, 8), (15, 16, 16))]
+    lo = [min(b[0][q] for b in boxes) for q in range(3)]
+    hi = [max(b[1][q] for b in boxes) for q in range(3)]
+    path = os.path.join(str(tmp_path), "union.obj")
+    write_obj(path, voxels_from_boxes(boxes, lo, hi), [lo[q] - nx // 2 for q in range(3)])
+    return path
+
+
+@pytest.mark.parametrize("shape", ["cube4", "union"])
+def test_cmatrix_device_build_is_inverse_of_exact_b(tmp_path, shape):
+    if shape == "cube4":
+        g = _oracle(cmatrix=False)
+    else:
+        nx = 25
+        cfg = _cfg(nx)
+        cfg["Object"]["Spacecraft"]["file_name"] = _union_obj(tmp_path, nx)
+        g = refcpu.Grid(oracle_params(cfg, nx=nx, ppc=1))
     ctx = F.Context(device_desc(g))
     nodes = g.cmat_nodes(0)
     ctx.set_object(0, g.cell_map(0), nodes)

@@ -196,3 +196,53 @@ def test_device_loader_at_256():
                 assert abs(m["var_v"][q] / sigma ** 2 - 1) < 5 * math.sqrt(2 / n) + 1e-6
     finally:
         h.close()
+
+
+def test_c4_learned_correction_against_the_warm_start():
+    """The learned-guess workload (128^3, the C2 plasma around its 16^3 cube,
+    FP32 particles) with the shipped network trained on warm-start pairs
+    (fopis_amd/weights/guess_c4_corr.npz, guess mode 2): the first solve of
+    each step starts from a relative residual at least ten times below the
+    warm start's (the reference warm-starts from the previous phi,
+    RootGridField.cpp:41-103), needs no more V-cycles, and ends at 1e-8. The
+    wall-clock comparison (inference included) is the bench's learned leg."""
+    import json
+
+    from fopis_amd.guess import Guess
+
+    w = os.path.join(os.path.dirname(CONFIGS), "weights", "guess_c4_corr.npz")
+    z = np.load(w, allow_pickle=False)
+    assert json.loads(str(z["recipe"]))["recipe"] == "correction"
+    with open(os.path.join(CONFIGS, "c2.json")) as f:
+        cfg = json.load(f)
+    cfg["Build"]["precision"] = "fp32"
+    for o in cfg["Object"].values():
+        o["file_name"] = os.path.join(CONFIGS, o["file_name"])
+    import tempfile
+
+    fd, path = tempfile.mkstemp(suffix=".json")
+    with os.fdopen(fd, "w") as f:
+        json.dump(cfg, f)
+    h = Host(path, device_loader=True, device_seed=4242, build_cmatrix=True, initialize=True, device_injection=True)
+    os.unlink(path)
+    try:
+        ctx = h.ctx
+        g = Guess(ctx, seed=1)
+        g.set_params(z["params"], float(z["out_scale"]))
+        for _ in range(2):
+            h.main_loop_es()
+        res = {}
+        for mode in (0, 2):
+            g.set_mode(mode)
+            ctx.reset_timers()
+            for _ in range(4):
+                h.main_loop_es()
+                assert F.solver_stats(ctx)["last_relres"] <= 1e-8
+            st = F.solver_stats(ctx)
+            res[mode] = (st["start_relres_first"], st["first_cycles"] / 4, ctx.phase_time("poisson_first")[0])
+        print(f"[c4] warm start: relres {res[0][0]:.3g}, {res[0][1]} cycles, {res[0][2]:.3f} ms; "
+              f"network: relres {res[2][0]:.3g}, {res[2][1]} cycles, {res[2][2]:.3f} ms (inference included)")
+        assert res[2][0] < 0.1 * res[0][0]
+        assert res[2][1] <= res[0][1]
+    finally:
+        h.close()

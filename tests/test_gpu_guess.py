@@ -284,3 +284,64 @@ def test_training_pairs_hdf5_round_trip(tmp_path):
     assert ga.eval(1, 4) == gb.eval(2, 5)
     a.close()
     b.close()
+
+
+def _lap7(u):
+    """6 u - sum of the six neighbours, zero outside (the solver's operator, dx = 1)."""
+    p = np.pad(u, 1)
+    return (6 * u - p[:-2, 1:-1, 1:-1] - p[2:, 1:-1, 1:-1] - p[1:-1, :-2, 1:-1] - p[1:-1, 2:, 1:-1]
+            - p[1:-1, 1:-1, :-2] - p[1:-1, 1:-1, 2:])
+
+
+def test_warm_start_correction_records_trains_and_lowers_the_start():
+    """Guess mode 3 records, from real PIC steps (33^3 nodes, 2 species, 16
+    ppc), pairs (residual charge of the warm start, potential the solve adds
+    to it); each pair satisfies A delta = r to the BF16 storage of its values
+    (checked here in numpy with the 7-point operator). The network trained on
+    them (mode 2: warm start plus its correction) starts the first solve of
+    later steps from a smaller residual than the warm start, and the solve
+    still ends at 1e-8. The reference warm-starts from the previous phi
+    (RootGridField.cpp:41-103)."""
+    n = 33
+    # 16 ppc of size 4 and eps0 = 6400: w_pe dt = 0.1 and a Debye length of one cell for v_th = 0.1
+    ctx = F.Context(F.make_desc(n, n, n, [(-1.0, 1.0, 4.0), (1.0, 100.0, 4.0)], eps0=6400.0, precision=F.FP32))
+    lo, hi = (0.0, 0.0, 0.0), (n - 1.0,) * 3
+    ctx.load_maxwellian(0, (n - 1) ** 3 * 16, 11, lo, hi, (0.0, 0.0, 0.0), 0.1)
+    ctx.load_maxwellian(1, (n - 1) ** 3 * 16, 12, lo, hi, (0.0, 0.0, 0.0), 0.01)
+    ctx.initialize()
+    g = GU.Guess(ctx, seed=5)
+    g.set_loss(1)
+    ntrain, nheld = 40, 8
+    g.dataset_alloc(ntrain + nheld)
+    for _ in range(2):
+        ctx.main_loop_es()
+    g.set_mode(3)
+    while g.recorded() < ntrain + nheld:
+        ctx.main_loop_es()
+    g.set_mode(0)
+    assert g.recorded() == ntrain + nheld
+    vn = n - 2  # voxels of interior nodes
+    for i in (0, ntrain + nheld - 1):
+        r, mask, d = g.dataset_get(i)
+        r = GU.from_bf16(r).astype(np.float64)[:vn, :vn, :vn]
+        d = GU.from_bf16(d).astype(np.float64)[:vn, :vn, :vn]
+        assert not mask.any()
+        assert abs(np.sqrt(np.mean(r ** 2)) - 1.0) < 1e-2  # normalised by its rms
+        err = np.linalg.norm(_lap7(d) - r) / np.linalg.norm(r)
+        assert err < 3e-2, err
+    before = g.eval(ntrain, ntrain + nheld)
+    g.train(ntrain, epochs=30, lr=2e-3, seed=3)
+    after = g.eval(ntrain, ntrain + nheld)
+    print(f"[guess] warm-start pairs: held-out loss {before:.4g} -> {after:.4g}")
+    assert after < 0.5 * before
+    starts = {}
+    for mode in (0, 2):
+        g.set_mode(mode)
+        ctx.reset_timers()
+        for _ in range(4):
+            ctx.main_loop_es()
+            assert F.solver_stats(ctx)["last_relres"] <= 1e-8
+        starts[mode] = F.solver_stats(ctx)["start_relres_first"]
+    print(f"[guess] first solve starts at relres {starts[0]:.3g} (warm) and {starts[2]:.3g} (warm + network)")
+    assert starts[2] < 0.5 * starts[0]
+    ctx.close()
