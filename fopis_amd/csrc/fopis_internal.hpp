@@ -185,7 +185,7 @@ struct SpeciesBins {
   DBuf<int64_t> ostart;  // [ntiles]
   DBuf<int32_t> ocapa;   // [ntiles]
   DBuf<int32_t> ocnt;    // [ntiles]
-  DBuf<int32_t> odcnt;   // [ntiles][27] outbox entries per direction
+  DBuf<int32_t> arr;     // [ntiles] arrivals reserved in each region during the arrivals pass (0 between steps)
   Soa<double> inbox;     // staged new particles (FP64 from the host), sorted by tile (grow-only)
   Soa<double> inbox_raw; // their upload in arrival order (grow-only)
   DBuf<int32_t> ibeg;    // [ntiles+1]

@@ -52,7 +52,6 @@ struct TileArgs {
   const int64_t* ostart;
   const int32_t* ocapa;
   int32_t* ocnt;
-  int32_t* odcnt;  // [ntiles][27] outbox entries per direction (written by the binning pass)
   const double* in[6];
   const int32_t* ibeg;
   SoaPtr<R> stray;
@@ -430,7 +429,6 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   __shared__ __attribute__((aligned(16))) char smem[SM::bytes];
   __shared__ int sh_rank[4 * kWaves];
   __shared__ int s_res[2];  // DB: slots reserved so far in the region and in the outbox
-  __shared__ int s_dir[(MODE & M_BIN) != 0 ? 27 : 1];  // outbox entries per direction
   R* sE = reinterpret_cast<R*>(smem);
   unsigned long long* sR = reinterpret_cast<unsigned long long*>(smem + SM::e_bytes);
   // the count window is its own LDS array so that its atomics stay LDS atomics (ds_add_u32)
@@ -465,7 +463,6 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     for (int q = threadIdx.x; q < kDW * kDW * kDW; q += kBlock) sD[q] = 0;
   }
   if constexpr ((MODE & M_BIN) != 0) {
-    if (threadIdx.x < 27) s_dir[threadIdx.x] = 0;
     if (threadIdx.x < 2) s_res[threadIdx.x] = 0;
   }
   __syncthreads();
@@ -866,7 +863,6 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           if (pos < ocap) {
             kind = 1;
             at = ost + pos;
-            atomicAdd(&s_dir[dir[u]], 1);
           } else {
             stray = true;
           }
@@ -919,9 +915,6 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     }
   }
   if constexpr (SM::kR || SM::kD || (MODE & M_BIN) != 0) __syncthreads();
-  if constexpr ((MODE & M_BIN) != 0) {
-    if (threadIdx.x < 27) a.odcnt[(size_t)27 * t + threadIdx.x] = s_dir[threadIdx.x];
-  }
   if constexpr (SM::kR) {
     double* P = a.rho_part + (size_t)t * (kRW * kRW * kRW);
     const double scale = a.q / (sizeof(R) == 4 ? (double)kFix32 : kFix);  // exact: powers of two
@@ -933,73 +926,61 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   }
 }
 
-// Arrivals: tile t appends the outbox entries its 26 neighbours addressed to
-// it, neighbour by neighbour in direction order, each in outbox order. The
-// binning pass counted every tile's outbox entries per direction, so one scan
-// over the 27 counts gives every direction its offset; each wavefront then
-// copies the matches of every kWaves-th direction in outbox order (ranks by
-// ballot), stopping once it has found them all.
+// Arrivals, from the sources: the workgroup of tile s sends every entry of
+// its outbox to the tile in the entry's direction, after that tile's
+// stayers. A wavefront reserves the slots of all its entries bound for one
+// tile with one atomic on that tile's arrival count (one per distinct
+// direction among its 64 entries), so each outbox is read once. Arrivals
+// past a tile's capacity go to the stray list. k_arrive_done then adds the
+// arrivals to the tiles' counts and clears them for the next step.
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_arrive(TileArgs<R> a) {
-  __shared__ int s_cnt[27];
-  __shared__ int s_off[28];
-  const int t = blockIdx.x;
-  const int tz = t % a.T[2], ty = (t / a.T[2]) % a.T[1], tx = t / (a.T[2] * a.T[1]);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+__global__ __launch_bounds__(kBlock) void k_scatter(TileArgs<R> a, int32_t* __restrict__ arr) {
+  const int s = blockIdx.x;
+  const int sz = s % a.T[2], sy = (s / a.T[2]) % a.T[1], sx = s / (a.T[2] * a.T[1]);
+  const int lane = threadIdx.x & 63;
   const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  // the source tile of direction d (or -1)
-  auto source = [&](int d) -> int {
-    if (d == 13) return -1;
-    const int sx = tx - (d / 9 - 1), sy = ty - ((d / 3) % 3 - 1), sz = tz - (d % 3 - 1);
-    if (sx < 0 || sy < 0 || sz < 0 || sx >= a.T[0] || sy >= a.T[1] || sz >= a.T[2]) return -1;
-    return (sx * a.T[1] + sy) * a.T[2] + sz;
-  };
-  if (threadIdx.x < 27) {
-    const int s = source(threadIdx.x);
-    s_cnt[threadIdx.x] = s < 0 ? 0 : a.odcnt[(size_t)27 * s + threadIdx.x];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int o = 0;
-    for (int d = 0; d < 27; ++d) {
-      s_off[d] = o;
-      o += s_cnt[d];
+  const int m = a.ocnt[s];
+  const int64_t ob = a.ostart[s];
+  for (int i0 = 0; i0 < m; i0 += kBlock) {
+    const int i = i0 + threadIdx.x;
+    const bool act = i < m;
+    const int d = act ? (int)a.odir[ob + i] : 13;
+    // the destination tile (the binning pass only files directions of existing neighbours)
+    const int t = ((sx + d / 9 - 1) * a.T[1] + (sy + (d / 3) % 3 - 1)) * a.T[2] + (sz + d % 3 - 1);
+    unsigned long long todo = __ballot(act);
+    int pos = 0;
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int dl = __shfl(d, leader, 64);
+      const unsigned long long grp = __ballot(act && d == dl) & todo;
+      int b = 0;
+      if (lane == leader) b = atomicAdd(&arr[__shfl(t, leader, 64)], __popcll(grp));
+      b = __shfl(b, leader, 64);
+      if (act && d == dl) pos = b + __popcll(grp & lt);
+      todo &= ~grp;
     }
-    s_off[27] = o;
-  }
-  __syncthreads();
-  const int n0 = a.cnt[t];
-  const int cap = a.capa[t];
-  const int64_t base = a.start[t];
-  for (int d = w; d < 27; d += kWaves) {
-    const int want = s_cnt[d];
-    if (want == 0) continue;
-    const int s = source(d);
-    const int m = a.ocnt[s];
-    const int64_t ob = a.ostart[s];
-    int off = n0 + s_off[d];
-    const int end = off + want;
-    for (int c0 = 0; c0 < m && off < end; c0 += 64) {
-      const int i = c0 + lane;
-      const bool hit = i < m && a.odir[ob + i] == (uint8_t)d;
-      const unsigned long long bm = __ballot(hit);
-      if (hit) {
-        R p[6];
+    if (act) {
+      R p[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) p[c] = a.out.c[c][ob + i];
-        const int pos = off + __popcll(bm & lt);
-        if (pos < cap) {
+      for (int c = 0; c < 6; ++c) p[c] = a.out.c[c][ob + i];
+      const int at = a.cnt[t] + pos;
+      if (at < a.capa[t]) {
+        const int64_t g = a.start[t] + at;
 #pragma unroll
-          for (int c = 0; c < 6; ++c) a.dst.c[c][base + pos] = p[c];
-        } else {
-          put_stray(a, p);
-        }
+        for (int c = 0; c < 6; ++c) a.dst.c[c][g] = p[c];
+      } else {
+        put_stray(a, p);
       }
-      off += __popcll(bm);
     }
   }
-  __syncthreads();  // every thread has read cnt[t] before it is replaced
-  if (threadIdx.x == 0) a.cnt[t] = min(n0 + s_off[27], cap);
+}
+
+__global__ void k_arrive_done(int32_t* __restrict__ cnt, const int32_t* __restrict__ capa, int32_t* __restrict__ arr,
+                              int nt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  cnt[t] = min(cnt[t] + arr[t], capa[t]);
+  arr[t] = 0;
 }
 
 // ------------------------------------------------------------- re-binning
@@ -1527,7 +1508,6 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.ostart = b.ostart.p;
   a.ocapa = b.ocapa.p;
   a.ocnt = b.ocnt.p;
-  a.odcnt = b.odcnt.p;
   for (int q = 0; q < 6; ++q) a.in[q] = b.inbox.c[q].p;
   a.ibeg = b.inbox_n > 0 ? b.ibeg.p : nullptr;
   a.stray = ptrs(b.stray);
@@ -1587,7 +1567,7 @@ void bin_flat(Ctx& c, int s, Soa<R>& flat, int64_t n, bool key_cell) {
     b.ostart.alloc(nt + 1);
     b.ocapa.alloc(nt + 1);
     b.ocnt.alloc(nt + 1);
-    b.odcnt.alloc((size_t)27 * (nt + 1));
+    b.arr.alloc(nt + 1);
     FOPIS_HIP(hipMemsetAsync(sc.capa.p + nt, 0, sizeof(uint32_t), st));
     FOPIS_HIP(hipMemsetAsync(sc.ocapa.p + nt, 0, sizeof(uint32_t), st));
     exclusive_scan_u32(st, sc.capa.p, nt + 1, b.start.p, sc.tmp);
@@ -1596,7 +1576,7 @@ void bin_flat(Ctx& c, int s, Soa<R>& flat, int64_t n, bool key_cell) {
     k_i32_from_u32<<<div_up(nt + 1, 256), 256, 0, st>>>(sc.ocapa.p, b.ocapa.p, nt + 1);
     k_i32_from_u32<<<div_up(nt + 1, 256), 256, 0, st>>>(sc.cnt.p, b.cnt.p, nt + 1);
     FOPIS_HIP(hipMemsetAsync(b.ocnt.p, 0, (nt + 1) * sizeof(int32_t), st));
-    FOPIS_HIP(hipMemsetAsync(b.odcnt.p, 0, (size_t)27 * (nt + 1) * sizeof(int32_t), st));
+    FOPIS_HIP(hipMemsetAsync(b.arr.p, 0, (nt + 1) * sizeof(int32_t), st));
     int64_t tot[2];
     FOPIS_HIP(hipMemcpyAsync(&tot[0], b.start.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     FOPIS_HIP(hipMemcpyAsync(&tot[1], b.ostart.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -1829,7 +1809,8 @@ void deposit_bin(Ctx& c, bool push) {
       auto& b = bins<R>(c)[s];
       TileArgs<R> a = tile_args<R>(c, s);
       if (dbs[s]) a.dst = ptrs(b.reg2);
-      k_arrive<R><<<c.ntiles, kBlock, 0, st>>>(a);
+      k_scatter<R><<<c.ntiles, kBlock, 0, st>>>(a, b.arr.p);
+      k_arrive_done<<<div_up(c.ntiles, 256), 256, 0, st>>>(b.cnt.p, b.capa.p, b.arr.p, c.ntiles);
       if (dbs[s]) b.reg.swap(b.reg2);
     }
     FOPIS_HIP(hipGetLastError());
