@@ -20,6 +20,8 @@ namespace fopis {
 // one workgroup owns a tile per launch and keeps the tile's E nodes and its
 // charge window in LDS.
 constexpr int kTile = 8;
+constexpr int kTileShift = 3;  // log2(kTile)
+static_assert((1 << kTileShift) == kTile, "kTile is a power of two");
 constexpr int kEW = kTile + 1;  // E-field window: nodes of the tile's cells
 constexpr int kRW = kTile + 3;  // charge window: tile cells +-1 after a move
 constexpr int kDW = kTile + 2;  // density window: cells of the tile +-1

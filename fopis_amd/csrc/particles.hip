@@ -41,7 +41,11 @@ struct TileArgs {
   int cx, cy, cz;  // node array dims with glue
   double xmax[3];  // largest valid coordinate, n - 1 (slx - dx)
   double qm, q, bx, by, bz, boris, size;
-  const double *ex, *ey, *ez;
+  // the same constants in the particles' precision (the push and the face
+  // test read these: FP32 kernels then hold them as single scalars)
+  R rqm, rb[3], rboris, rxmax[3];
+  const double* ex;  // nodal E: component c at ex + c * e_s
+  int64_t e_s;
   SoaPtr<R> reg;
   SoaPtr<R> dst;   // regions the binning pass writes (the other buffer of a double-buffered set, or reg)
   const int64_t* start;
@@ -52,7 +56,8 @@ struct TileArgs {
   const int64_t* ostart;
   const int32_t* ocapa;
   int32_t* ocnt;
-  const double* in[6];
+  const double* in;  // staged new particles: component c at in + c * in_s
+  int64_t in_s;
   const int32_t* ibeg;
   SoaPtr<R> stray;
   int64_t stray_cap;
@@ -429,6 +434,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   __shared__ __attribute__((aligned(16))) char smem[SM::bytes];
   __shared__ int sh_rank[4 * kWaves];
   __shared__ int s_res[2];  // DB: slots reserved so far in the region and in the outbox
+  // where the binning pass stores a particle of each kind (0 tile region, 1
+  // outbox, 2 stray list, 3 sink): the set's base address, its component
+  // stride and the base of the direction byte, all in bytes; one LDS read per
+  // particle replaces a four-way choice among twelve 64-bit scalars
+  __shared__ __attribute__((aligned(16))) unsigned long long s_dst[(MODE & M_BIN) != 0 ? 4 : 1][4];
   R* sE = reinterpret_cast<R*>(smem);
   unsigned long long* sR = reinterpret_cast<unsigned long long*>(smem + SM::e_bytes);
   // the count window is its own LDS array so that its atomics stay LDS atomics (ds_add_u32)
@@ -447,8 +457,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
       if (gi < a.cx && gj < a.cy && gk < a.cz) {
         const size_t g = ((size_t)gi * a.cy + gj) * a.cz + gk;
         vx = (R)a.ex[g];
-        vy = (R)a.ey[g];
-        vz = (R)a.ez[g];
+        vy = (R)a.ex[g + a.e_s];
+        vz = (R)a.ex[g + 2 * a.e_s];
       }
       sE[4 * q] = vx;
       sE[4 * q + 1] = vy;
@@ -464,6 +474,15 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   }
   if constexpr ((MODE & M_BIN) != 0) {
     if (threadIdx.x < 2) s_res[threadIdx.x] = 0;
+    if (threadIdx.x < 4) {
+      const int k = threadIdx.x;
+      const SoaPtr<R>& set = k == 0 ? a.dst : (k == 1 ? a.out : a.stray);
+      const uint64_t snk = reinterpret_cast<uint64_t>(a.sink);
+      s_dst[k][0] = k == 3 ? snk : reinterpret_cast<uint64_t>(set.c[0]);
+      s_dst[k][1] = k == 3 ? 0ull : reinterpret_cast<uint64_t>(set.c[1]) - reinterpret_cast<uint64_t>(set.c[0]);
+      s_dst[k][2] = k == 1 ? reinterpret_cast<uint64_t>(a.odir) : snk;
+      s_dst[k][3] = 0ull;
+    }
   }
   __syncthreads();
 
@@ -488,15 +507,6 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   const int64_t ost = ((MODE & M_BIN) != 0) ? a.ostart[t] : 0;
   const int rcap = a.capa[t];
   const int64_t sink_at = ((int64_t)t * kBlock + threadIdx.x) & (kSink - 1);
-  // destinations of the binning stores (byte addresses and component strides)
-  const uint64_t dst_b = reinterpret_cast<uint64_t>(a.dst.c[0]);
-  const uint64_t dst_s = reinterpret_cast<uint64_t>(a.dst.c[1]) - dst_b;
-  const uint64_t out_b = reinterpret_cast<uint64_t>(a.out.c[0]);
-  const uint64_t out_s = reinterpret_cast<uint64_t>(a.out.c[1]) - out_b;
-  const uint64_t str_b = reinterpret_cast<uint64_t>(a.stray.c[0]);
-  const uint64_t str_s = reinterpret_cast<uint64_t>(a.stray.c[1]) - str_b;
-  const uint64_t snk_b = reinterpret_cast<uint64_t>(a.sink);
-  const uint64_t odir_b = reinterpret_cast<uint64_t>(a.odir);
 
   // PPT particles per thread and iteration (FP32: two chunks of kBlock, whose
   // loads are in flight together; one block scan ranks both), the next
@@ -509,14 +519,21 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   // and a 32-bit element index: loads take the scalar base plus a vector offset
   const R* const rb = a.reg.c[0] + base;
   const int64_t rs = a.reg.c[1] - a.reg.c[0];
-  auto fetch = [&](int idx, R* q) {
-    const unsigned j = idx < nreg ? (unsigned)idx : 0u;
+  const __attribute__((address_space(1))) R* rbc[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) q[c] = sgpr_ptr(rb + c * rs)[j];
+  for (int c = 0; c < 6; ++c) rbc[c] = sgpr_ptr(rb + c * rs);
+  auto fetch = [&](int idx, R* q) {
+    // a 32-bit byte offset (a tile region holds far fewer than 2^29 particles),
+    // so that each load is the scalar base plus one vector offset register
+    const uint32_t jb = (idx < nreg ? (uint32_t)idx : 0u) * (uint32_t)sizeof(R);
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      q[c] = *reinterpret_cast<const __attribute__((address_space(1))) R*>(
+          reinterpret_cast<const __attribute__((address_space(1))) char*>(rbc[c]) + jb);
     if (idx >= nreg) {
       if (idx < ntot) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) q[c] = (R)a.in[c][ib0 + idx - nreg];
+        for (int c = 0; c < 6; ++c) q[c] = (R)a.in[c * a.in_s + ib0 + idx - nreg];
       } else {
 #pragma unroll
         for (int c = 0; c < 6; ++c) q[c] = (R)0;
@@ -546,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           const int ci = (int)fx, cj = (int)fy, ck = (int)fz;
           const float dx1 = p[0] - fx, dy1 = p[1] - fy, dz1 = p[2] - fz;
           const float dx2 = 1.0f - dx1, dy2 = 1.0f - dy1, dz2 = 1.0f - dz1;
-          const float qm = (float)a.qm;
+          const float qm = a.rqm;
           const float qz1 = qm * dz1, qz2 = qm * dz2;
           const float a22 = dx2 * dy2, a12 = dx1 * dy2, a21 = dx2 * dy1, a11 = dx1 * dy1;
           const float v1 = a22 * qz2, v2 = a12 * qz2, v3 = a21 * qz2, v4 = a11 * qz2;
@@ -567,15 +584,14 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           } else {  // outside the tile (a bin left stale by an interrupted step): global gather
             const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
             const size_t g = (size_t)(ci + 1) * sI + (size_t)(cj + 1) * sJ + (ck + 1);
-            const double* Es[3] = {a.ex, a.ey, a.ez};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-              const double* E = Es[c] + g;
+              const double* E = a.ex + c * a.e_s + g;
               e[c] = v1 * (float)E[0] + v2 * (float)E[sI] + v3 * (float)E[sJ] + v4 * (float)E[sI + sJ] +
                      v5 * (float)E[1] + v6 * (float)E[sI + 1] + v7 * (float)E[sJ + 1] + v8 * (float)E[sI + sJ + 1];
             }
           }
-          const float bx = (float)a.bx, by = (float)a.by, bz = (float)a.bz, boris = (float)a.boris;
+          const float bx = a.rb[0], by = a.rb[1], bz = a.rb[2], boris = a.rboris;
           const float vx1 = p[3] + e[0], vy1 = p[4] + e[1], vz1 = p[5] + e[2];
           const float vxt = __builtin_fmaf(vy1, bz, __builtin_fmaf(-vz1, by, vx1));
           const float vyt = __builtin_fmaf(vz1, bx, __builtin_fmaf(-vx1, bz, vy1));
@@ -593,7 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           ck = min(ck, a.N[2] - 1);
           const R dx1 = x - (R)ci, dy1 = y - (R)cj, dz1 = z - (R)ck;
           const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
-          const R qm = (R)a.qm;
+          const R qm = a.rqm;
           const R v1 = qm * dx2 * dy2 * dz2;
           const R v2 = qm * dx1 * dy2 * dz2;
           const R v3 = qm * dx2 * dy1 * dz2;
@@ -617,15 +633,14 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           } else {  // outside the tile (a bin left stale by an interrupted step): global gather
             const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
             const size_t g = (size_t)(ci + 1) * sI + (size_t)(cj + 1) * sJ + (ck + 1);
-            const double* Es[3] = {a.ex, a.ey, a.ez};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-              const double* E = Es[c] + g;
+              const double* E = a.ex + c * a.e_s + g;
               e[c] = v1 * (R)E[0] + v2 * (R)E[sI] + v3 * (R)E[sJ] + v4 * (R)E[sI + sJ] + v5 * (R)E[1] +
                      v6 * (R)E[sI + 1] + v7 * (R)E[sJ + 1] + v8 * (R)E[sI + sJ + 1];
             }
           }
-          const R bx = (R)a.bx, by = (R)a.by, bz = (R)a.bz, boris = (R)a.boris;
+          const R bx = a.rb[0], by = a.rb[1], bz = a.rb[2], boris = a.rboris;
           const R vx1 = p[3] + e[0], vy1 = p[4] + e[1], vz1 = p[5] + e[2];
           const R vxt = vx1 + vy1 * bz - vz1 * by;
           const R vyt = vy1 + vz1 * bx - vx1 * bz;
@@ -642,8 +657,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           p[0] = p[0] + p[3];
           p[1] = p[1] + p[4];
           p[2] = p[2] + p[5];
-          if (p[0] < (R)0 || p[0] > (R)a.xmax[0] || p[1] < (R)0 || p[1] > (R)a.xmax[1] || p[2] < (R)0 ||
-              p[2] > (R)a.xmax[2]) {
+          if (p[0] < (R)0 || p[0] > a.rxmax[0] || p[1] < (R)0 || p[1] > a.rxmax[1] || p[2] < (R)0 ||
+              p[2] > a.rxmax[2]) {
             valid = false;
             lost = true;
           }
@@ -657,8 +672,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         ci = (int)floor(p[0]);
         cj = (int)floor(p[1]);
         ck = (int)floor(p[2]);
-        if (from_in && (p[0] < (R)0 || p[0] > (R)a.xmax[0] || p[1] < (R)0 || p[1] > (R)a.xmax[1] ||
-                        p[2] < (R)0 || p[2] > (R)a.xmax[2]))
+        if (from_in && (p[0] < (R)0 || p[0] > a.rxmax[0] || p[1] < (R)0 || p[1] > a.rxmax[1] ||
+                        p[2] < (R)0 || p[2] > a.rxmax[2]))
           valid = false;
       }
 
@@ -791,8 +806,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         dir[u] = 13;
         far[u] = false;
         if (valid[u]) {
-          const int bi = min(ci[u], a.N[0] - 1) / kTile, bj = min(cj[u], a.N[1] - 1) / kTile,
-                    bk = min(ck[u], a.N[2] - 1) / kTile;
+          // cells of valid particles are >= 0: a shift is the division
+          const int bi = min(ci[u], a.N[0] - 1) >> kTileShift, bj = min(cj[u], a.N[1] - 1) >> kTileShift,
+                    bk = min(ck[u], a.N[2] - 1) >> kTileShift;
           const int d0 = bi - tx, d1 = bj - ty, d2 = bk - tz;
           if (d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1)
             far[u] = true;
@@ -879,15 +895,14 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         // one base address and one component stride per destination: each
         // particle set is one allocation with its components a fixed stride
         // apart, and the sink takes all six components in one slot
-        const uint64_t db = kind == 0 ? dst_b : (kind == 1 ? out_b : (kind == 2 ? str_b : snk_b));
-        const uint64_t ds = kind == 0 ? dst_s : (kind == 1 ? out_s : (kind == 2 ? str_s : 0ull));
+        const uint64_t db = s_dst[kind][0], ds = s_dst[kind][1], dd = s_dst[kind][2];
         uint64_t ad = db + (uint64_t)at * sizeof(R);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
           *reinterpret_cast<__attribute__((address_space(1))) R*>(ad) = p[u][c];
           ad += ds;
         }
-        const uint64_t od = kind == 1 ? odir_b + (uint64_t)at : snk_b + (uint64_t)sink_at;
+        const uint64_t od = dd + (uint64_t)(kind == 1 ? at : sink_at);
         *reinterpret_cast<__attribute__((address_space(1))) uint8_t*>(od) = (uint8_t)dir[u];
         wb += ts[u];
         ob0 += tm[u];
@@ -1496,8 +1511,13 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.bz = c.d.static_b[2];
   a.boris = 2.0 / (1.0 + (a.bx * a.bx + a.by * a.by + a.bz * a.bz));
   a.ex = c.eref.p;
-  a.ey = c.eref.p + c.nnode;
-  a.ez = c.eref.p + 2 * c.nnode;
+  a.e_s = (int64_t)c.nnode;
+  a.rqm = (R)a.qm;
+  a.rb[0] = (R)a.bx;
+  a.rb[1] = (R)a.by;
+  a.rb[2] = (R)a.bz;
+  a.rboris = (R)a.boris;
+  for (int q = 0; q < 3; ++q) a.rxmax[q] = (R)a.xmax[q];
   a.reg = ptrs(b.reg);
   a.dst = a.reg;
   a.start = b.start.p;
@@ -1508,7 +1528,8 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.ostart = b.ostart.p;
   a.ocapa = b.ocapa.p;
   a.ocnt = b.ocnt.p;
-  for (int q = 0; q < 6; ++q) a.in[q] = b.inbox.c[q].p;
+  a.in = b.inbox.c[0].p;
+  a.in_s = b.inbox.c[1].p - b.inbox.c[0].p;
   a.ibeg = b.inbox_n > 0 ? b.ibeg.p : nullptr;
   a.stray = ptrs(b.stray);
   a.stray_cap = b.stray_cap;
