@@ -245,19 +245,25 @@ def pmc_traffic(workload: str):
     return None
 
 
-def poisson_roofline(desc, ms, cycles):
+def poisson_roofline(desc, ms, cycles, updates=None, pair_passes=0.0):
     """Algorithmic HBM bytes of one Poisson solve over its measured time. Per
     fine-grid unknown: a V(2,2) cycle moves 83 B with FP32 levels (166 B with
     FP64: four smoothing sweeps of 12 B plus residual, restriction and
-    prolongation, and 8/7 over the coarser levels), the FP64 update of the
-    defect correction 40 B per cycle, and the two residuals formed from
-    scratch (start and end) 28 B each."""
+    prolongation, and 8/7 over the coarser levels), each FP64 update of the
+    defect correction 40 B (one per cycle, or one per pair of cycles sharing a
+    right-hand side), each residual pass between the cycles of a pair 8 B
+    (16 B with FP64 levels), and the two residuals formed from scratch (start
+    and end) 28 B each. updates and pair_passes are per solve."""
     n = (desc.nx - 2) * (desc.ny - 2) * (desc.nz - 2)
     vb = 83 if desc.mg_precision == 1 else 166
-    nbytes = n * (cycles * (vb + 40) + 2 * 28)
+    pb = 8 if desc.mg_precision == 1 else 16
+    if updates is None:
+        updates = cycles
+    nbytes = n * (cycles * vb + updates * 40 + pair_passes * pb + 2 * 28)
     gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-            "algorithmic_bytes_per_solve": nbytes, "fine_unknowns": n}
+            "algorithmic_bytes_per_solve": nbytes, "fine_unknowns": n, "cycles_per_solve": cycles,
+            "updates_per_solve": updates, "pair_passes_per_solve": pair_passes}
 
 
 def main():
@@ -386,7 +392,9 @@ def main():
         "poisson_first_solve_cycles": stats["first_cycles"] / max(1, args.steps),
         "efield_ms": ef_ms,
         "poisson_roofline": poisson_roofline(h.desc, poi_ms,
-                                             stats["total_cycles"] / max(1, stats["total_solves"])),
+                                             stats["total_cycles"] / max(1, stats["total_solves"]),
+                                             stats["updates"] / max(1, stats["total_solves"]),
+                                             stats["pair_passes"] / max(1, stats["total_solves"])),
         "host_generate_ms_per_step": gen_ms / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -480,6 +480,8 @@ int fopis_reset_timers(fopis_ctx* ctx) {
     ctx->c.timers.clear();
     ctx->c.mg_cycles_total = 0;
     ctx->c.mg_solves_total = 0;
+    ctx->c.mg_updates_total = 0;
+    ctx->c.mg_pair_passes_total = 0;
     ctx->c.mg_first_cycles_total = 0;
     ctx->c.relres0_first = ctx->c.relres0_other = 0.0;
     ctx->c.n_first = ctx->c.n_other = 0;
@@ -505,6 +507,18 @@ int fopis_moments(fopis_ctx* ctx, int s, double* out) {
     return FOPIS_E_ARG;
   }
   return guarded([&] { particles_moments(ctx->c, s, out); });
+}
+
+int fopis_solver_passes(fopis_ctx* ctx, int64_t* out) {
+  NEED_CTX
+  if (!out) {
+    g_last_error = "null argument";
+    return FOPIS_E_ARG;
+  }
+  return guarded([&] {
+    out[0] = ctx->c.mg_updates_total;
+    out[1] = ctx->c.mg_pair_passes_total;
+  });
 }
 
 int fopis_first_solve_cycles(fopis_ctx* ctx, int64_t* total_cycles) {

@@ -289,6 +289,9 @@ struct Ctx {
   std::vector<MgLevel> mg;
   int mg_cycles_last = 0;
   int64_t mg_cycles_total = 0, mg_solves_total = 0;
+  // FP64 defect updates and pair residual passes (two cycles share an update)
+  int mg_updates_last = 0, mg_pair_passes_last = 0;
+  int64_t mg_updates_total = 0, mg_pair_passes_total = 0;
   int64_t mg_first_cycles_total = 0;  // cycles of the first solve of each step
   double relres_last = 0;
   double relres0_last = 0;     // relative residual the last solve started from
@@ -299,10 +302,13 @@ struct Ctx {
   bool carry = true;           // warm starts from psi (FOPIS_CARRY=0: the previous phi for both solves)
   DBuf<int32_t> mg_state;     // [4] solve flag and cycle count on the device
   DBuf<double> mg_part;       // per-block residual partials and the norms of the solve (fixed address)
-  hipGraphExec_t mg_graph = nullptr;  // one V-cycle + FP64 update + norm check, captured once
-  size_t mg_graph_esz = 0;    // element size the graph was captured for
-  double mg_graph_tol = 0.0;  // tolerance and cycle cap captured in its norm kernel
+  // captured once: [0] one V-cycle + FP64 update + norm check, [1] two
+  // V-cycles (the second continuing from the first's correction) + update + check
+  hipGraphExec_t mg_graph[2] = {nullptr, nullptr};
+  size_t mg_graph_esz = 0;    // element size the graphs were captured for
+  double mg_graph_tol = 0.0;  // tolerance and cycle cap captured in their norm kernels
   int mg_graph_cap = 0;
+  bool mg_pairs = true;  // a solve's predicted cycles run two per FP64 update
   bool mg_use_graph = true;   // replay the captured cycle (FOPIS_MG_GRAPH=0 launches it kernel by kernel)
   DBuf<double> mg_r64;        // FP64 level-0 residual when the V-cycle is FP32
   int mg_hist[2] = {1, 1};    // cycles of the last two solves (batch size of the next)

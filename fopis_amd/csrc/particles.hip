@@ -447,6 +447,10 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   const int t = blockIdx.x;
   const int tz = t % a.T[2], ty = (t / a.T[2]) % a.T[1], tx = t / (a.T[2] * a.T[1]);
   const int o0 = tx * kTile, o1 = ty * kTile, o2 = tz * kTile;
+  const int ok[3] = {o0, o1, o2};
+  // per axis: the tile's origin minus one, and the cells from the origin to the grid's end
+  const int om1[3] = {o0 - 1, o1 - 1, o2 - 1};
+  const int nmo[3] = {a.N[0] - o0, a.N[1] - o1, a.N[2] - o2};
 
   if constexpr (SM::kE) {
     constexpr int ne = kEW * kEW * kEW;
@@ -545,8 +549,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   for (int u = 0; u < PPT; ++u) fetch(u * kBlock + threadIdx.x, nxt[u]);
 
   // one particle through velocity, position, cell, density, capture and
-  // deposit; returns whether it is still valid (its cell in ci, cj, ck)
-  auto work = [&](R (&p)[6], int idx, int& ci, int& cj, int& ck, bool& lost) -> bool {
+  // deposit; returns whether it is still valid (and in dt the tile it bins
+  // into, as steps 0, 1, 2 per axis from the neighbour below)
+  auto work = [&](R (&p)[6], int idx, int (&dt)[3], bool& lost) -> bool {
     const bool act = idx < ntot;
     const bool from_in = idx >= nreg;
     bool valid = act && !isnan(p[0]);
@@ -657,33 +662,47 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           p[0] = p[0] + p[3];
           p[1] = p[1] + p[4];
           p[2] = p[2] + p[5];
-          if (p[0] < (R)0 || p[0] > a.rxmax[0] || p[1] < (R)0 || p[1] > a.rxmax[1] || p[2] < (R)0 ||
-              p[2] > a.rxmax[2]) {
-            valid = false;
-            lost = true;
-          }
         }
       }
-
-      ci = 0;
-      cj = 0;
-      ck = 0;
-      if (valid) {
-        ci = (int)floor(p[0]);
-        cj = (int)floor(p[1]);
-        ck = (int)floor(p[2]);
-        if (from_in && (p[0] < (R)0 || p[0] > a.rxmax[0] || p[1] < (R)0 || p[1] > a.rxmax[1] ||
-                        p[2] < (R)0 || p[2] > a.rxmax[2]))
-          valid = false;
+      // outside the domain (a pushed particle is absorbed there, a staged one is dropped)
+      const bool out = (p[0] < (R)0) | (p[0] > a.rxmax[0]) | (p[1] < (R)0) | (p[1] > a.rxmax[1]) |
+                       (p[2] < (R)0) | (p[2] > a.rxmax[2]);
+      if constexpr ((MODE & M_POS) != 0) {
+        lost = valid & !from_in & out;
+        valid = valid & !out;
+      } else {
+        valid = valid & !(from_in & out);
       }
+
+      // the cell (floor of the position), and the same cell counted from the
+      // tile's origin minus one (w); the deposit's cell d = min(cell, N - 1)
+      // keeps the upper face's particles in the last cell (as the reference's
+      // weights do), wd being w clamped the same way. Density window, charge
+      // window and conductor test share one predicate: every axis of w in
+      // [0, 10), the tile's cells and one more on each side (for a valid
+      // particle, w in [0, 10) if and only if wd is).
+      R fl[3];
+      int w[3], wd[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        fl[k] = floor(p[k]);
+        w[k] = (int)fl[k] - om1[k];
+        wd[k] = min(w[k], nmo[k]);
+      }
+      const bool near = max(max((unsigned)w[0], (unsigned)w[1]), (unsigned)w[2]) < (unsigned)kDW;
+      static_assert(kDW == kRW - 1, "the density and charge windows cover the same cells");
+      // this tile's neighbour the particle bins into, as steps -1, 0, +1 per
+      // axis plus one (floor division of the clamped cell by the tile width)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dt[k] = (wd[k] + kTile - 1) >> kTileShift;
 
       // ---- density: count per reference cell (RootGridField.cpp:1129-1150)
       if constexpr (SM::kD) {
         if (valid) {
-          const int wi = ci - o0 + 1, wj = cj - o1 + 1, wk = ck - o2 + 1;
-          if (wi >= 0 && wi < kDW && wj >= 0 && wj < kDW && wk >= 0 && wk < kDW) {
-            atomicAdd(&sD[(wi * kDW + wj) * kDW + wk], 1);
+          if (near) {
+            atomicAdd(&sD[(w[0] * kDW + w[1]) * kDW + w[2]], 1);
           } else {
+            const int ci = w[0] + o0 - 1, cj = w[1] + o1 - 1, ck = w[2] + o2 - 1;
             const size_t g = ((size_t)(ci + 1) * (a.cy - 1) + (cj + 1)) * (a.cz - 1) + (ck + 1);
             atomicAdd(&a.dens_g[g], a.size);
             atomicAdd(&a.status[2], 1);
@@ -694,11 +713,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
       // ---- conductor capture, then CIC deposit
       if constexpr ((MODE & M_DEP) != 0) {
         // a tile with no conductor cell within one tile of it reads no mask for
-        // particles that end within that box
-        const bool in_box = ci >= o0 - kTile && ci < o0 + 2 * kTile && cj >= o1 - kTile && cj < o1 + 2 * kTile &&
-                            ck >= o2 - kTile && ck < o2 + 2 * kTile;
-        if (valid && a.nobj > 0 && (near_obj || !in_box)) {
-          const int mi = ci + 1, mj = cj + 1, mk = ck + 1;
+        // particles that end next to it
+        if (valid && a.nobj > 0 && (near_obj || !near)) {
+          const int mi = w[0] + o0, mj = w[1] + o1, mk = w[2] + o2;
           const uint8_t m = a.mask[((size_t)mi * (a.cy - 1) + mj) * (a.cz - 1) + mk];
           if (m != 0) {
             // captured: recorded for the capture kernel (surface charge on the device)
@@ -715,16 +732,15 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           }
         }
         if (valid) {
-          const int di = min(ci, a.N[0] - 1), dj = min(cj, a.N[1] - 1), dk = min(ck, a.N[2] - 1);
-          const R dx1 = p[0] - (R)di, dy1 = p[1] - (R)dj, dz1 = p[2] - (R)dk;
+          const R dx1 = p[0] - fmin(fl[0], (R)(a.N[0] - 1)), dy1 = p[1] - fmin(fl[1], (R)(a.N[1] - 1)),
+                  dz1 = p[2] - fmin(fl[2], (R)(a.N[2] - 1));
           const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
           // CIC weights of the eight nodes (the particle's charge is applied when
           // the window is written out)
-          const R w[8] = {dx2 * dy2 * dz2, dx1 * dy2 * dz2, dx2 * dy1 * dz2, dx1 * dy1 * dz2,
-                          dx2 * dy2 * dz1, dx1 * dy2 * dz1, dx2 * dy1 * dz1, dx1 * dy1 * dz1};
-          const int wi = di - o0 + 1, wj = dj - o1 + 1, wk = dk - o2 + 1;
-          const bool in_win = wi >= 0 && wi + 1 < kRW && wj >= 0 && wj + 1 < kRW && wk >= 0 && wk + 1 < kRW;
-          if (sizeof(R) == 4 && in_win) {
+          const R w8[8] = {dx2 * dy2 * dz2, dx1 * dy2 * dz2, dx2 * dy1 * dz2, dx1 * dy1 * dz2,
+                           dx2 * dy2 * dz1, dx1 * dy2 * dz1, dx2 * dy1 * dz1, dx1 * dy1 * dz1};
+          const int wi = wd[0], wj = wd[1], wk = wd[2];
+          if (sizeof(R) == 4 && near) {
             // FP32 particles: kFix32 units per weight, the factored weights
             // scaled and rounded in one multiply-add each (+0.5, then the
             // conversion truncates), 32-bit values into the 64-bit window
@@ -741,32 +757,33 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
             atomicAdd(W + kRW * kRW + 1, fx(__builtin_fmaf(a12, s1, 0.5f)));
             atomicAdd(W + kRW + 1, fx(__builtin_fmaf(a21, s1, 0.5f)));
             atomicAdd(W + kRW * kRW + kRW + 1, fx(__builtin_fmaf(a11, s1, 0.5f)));
-          } else if (sizeof(R) == 8 && fix_ok && in_win) {
+          } else if (sizeof(R) == 8 && fix_ok && near) {
             // 64-bit fixed point (kFix units per weight 1): integer sums do not
             // depend on the order the atomics land in, so the window, and rho,
             // are the same bit for bit from run to run
             unsigned long long* W = sR + (wi * kRW + wj) * kRW + wk;
             auto fx = [](R v) -> unsigned long long { return fix_weight((double)v); };
-            atomicAdd(W, fx(w[0]));
-            atomicAdd(W + kRW * kRW, fx(w[1]));
-            atomicAdd(W + kRW, fx(w[2]));
-            atomicAdd(W + kRW * kRW + kRW, fx(w[3]));
-            atomicAdd(W + 1, fx(w[4]));
-            atomicAdd(W + kRW * kRW + 1, fx(w[5]));
-            atomicAdd(W + kRW + 1, fx(w[6]));
-            atomicAdd(W + kRW * kRW + kRW + 1, fx(w[7]));
+            atomicAdd(W, fx(w8[0]));
+            atomicAdd(W + kRW * kRW, fx(w8[1]));
+            atomicAdd(W + kRW, fx(w8[2]));
+            atomicAdd(W + kRW * kRW + kRW, fx(w8[3]));
+            atomicAdd(W + 1, fx(w8[4]));
+            atomicAdd(W + kRW * kRW + 1, fx(w8[5]));
+            atomicAdd(W + kRW + 1, fx(w8[6]));
+            atomicAdd(W + kRW * kRW + kRW + 1, fx(w8[7]));
           } else {
+            const int di = wi + o0 - 1, dj = wj + o1 - 1, dk = wk + o2 - 1;
             const double q = a.q;
             const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
             double* G = a.rho_g + (size_t)(di + 1) * sI + (size_t)(dj + 1) * sJ + (dk + 1);
-            atomicAdd(G, q * (double)w[0]);
-            atomicAdd(G + sI, q * (double)w[1]);
-            atomicAdd(G + sJ, q * (double)w[2]);
-            atomicAdd(G + sI + sJ, q * (double)w[3]);
-            atomicAdd(G + 1, q * (double)w[4]);
-            atomicAdd(G + sI + 1, q * (double)w[5]);
-            atomicAdd(G + sJ + 1, q * (double)w[6]);
-            atomicAdd(G + sI + sJ + 1, q * (double)w[7]);
+            atomicAdd(G, q * (double)w8[0]);
+            atomicAdd(G + sI, q * (double)w8[1]);
+            atomicAdd(G + sJ, q * (double)w8[2]);
+            atomicAdd(G + sI + sJ, q * (double)w8[3]);
+            atomicAdd(G + 1, q * (double)w8[4]);
+            atomicAdd(G + sI + 1, q * (double)w8[5]);
+            atomicAdd(G + sJ + 1, q * (double)w8[6]);
+            atomicAdd(G + sI + sJ + 1, q * (double)w8[7]);
             atomicAdd(&a.status[2], 1);
           }
         }
@@ -784,11 +801,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
 #pragma unroll
     for (int u = 0; u < PPT; ++u) fetch(c0 + (PPT + u) * kBlock + threadIdx.x, nxt[u]);
     bool valid[PPT], lost[PPT];
-    int ci[PPT], cj[PPT], ck[PPT];
+    int dt[PPT][3];
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
       lost[u] = false;
-      valid[u] = work(p[u], c0 + u * kBlock + threadIdx.x, ci[u], cj[u], ck[u], lost[u]);
+      valid[u] = work(p[u], c0 + u * kBlock + threadIdx.x, dt[u], lost[u]);
     }
     if constexpr ((MODE & M_POS) != 0) {
       // particles absorbed at a domain face, one atomic per wavefront
@@ -806,14 +823,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         dir[u] = 13;
         far[u] = false;
         if (valid[u]) {
-          // cells of valid particles are >= 0: a shift is the division
-          const int bi = min(ci[u], a.N[0] - 1) >> kTileShift, bj = min(cj[u], a.N[1] - 1) >> kTileShift,
-                    bk = min(ck[u], a.N[2] - 1) >> kTileShift;
-          const int d0 = bi - tx, d1 = bj - ty, d2 = bk - tz;
-          if (d0 < -1 || d0 > 1 || d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1)
+          // one of the 27 tiles around this one (13: this tile), or farther
+          if (max(max((unsigned)dt[u][0], (unsigned)dt[u][1]), (unsigned)dt[u][2]) > 2u)
             far[u] = true;
           else
-            dir[u] = (d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1);
+            dir[u] = (dt[u][0] * 3 + dt[u][1]) * 3 + dt[u][2];
         }
         stay[u] = valid[u] && !far[u] && dir[u] == 13;
         move[u] = valid[u] && !far[u] && dir[u] != 13;
@@ -821,17 +835,20 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
       int rs[PPT], rm[PPT], ts[PPT], tm[PPT];
       if constexpr (DB) {
         // slots of this wavefront: its stayers and movers in (u, lane) order
-        // after one reservation each in the region and the outbox
+        // after one reservation each in the region and the outbox; below(m)
+        // counts the lanes of mask m below this one (v_mbcnt)
+        auto below = [](unsigned long long m) -> int {
+          return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        };
         const int lane = threadIdx.x & 63;
-        const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
         unsigned long long ms[PPT], mm[PPT];
         int ns = 0, nm = 0;
 #pragma unroll
         for (int u = 0; u < PPT; ++u) {
           ms[u] = __ballot(stay[u]);
           mm[u] = __ballot(move[u]);
-          rs[u] = ns + __popcll(ms[u] & lt);
-          rm[u] = nm + __popcll(mm[u] & lt);
+          rs[u] = ns + below(ms[u]);
+          rm[u] = nm + below(mm[u]);
           ns += __popcll(ms[u]);
           nm += __popcll(mm[u]);
         }

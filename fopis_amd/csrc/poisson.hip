@@ -95,11 +95,16 @@ __global__ __launch_bounds__(256) void k_resid0(const double* __restrict__ phi, 
   }
 }
 
-// Solve state on the device: st[0] done flag, st[1] cycles applied;
-// dst[0..2] = ||r||^2, ||b||^2, relative residual of the last check.
+// Solve state on the device: st[0] done flag, st[1] cycles applied, st[3]
+// stays 0 (the done flag of launches that always run), st[4] the current
+// pair's skip flag, st[5] FP64 updates applied, st[6] pair residual passes
+// run; dst[0..2] = ||r||^2, ||b||^2, relative residual of the
+// last check, dst[3] the relative residual the solve started from.
 __global__ void k_solve_begin(int* st) {
   st[0] = 0;
   st[1] = 0;
+  st[5] = 0;
+  st[6] = 0;
 }
 
 // After a V-cycle's correction e: phi += e and the FP64 residual
@@ -142,11 +147,73 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ phi, const 
   }
 }
 
+// Between the two V-cycles of a pair: per-block partial sums of t^2, t = r - A e
+// at the level-0 interior (the cycles' right-hand side and the first cycle's
+// correction, summed in FP64; the b^2 slot is 0).
+template <class P>
+__global__ __launch_bounds__(256) void k_pair_resid(const P* __restrict__ e, const P* __restrict__ r, Lv L,
+                                                    double* part, const int* __restrict__ done) {
+  if (*done) return;
+  __shared__ double s1[256];
+  const int64_t nint = (int64_t)L.n0 * L.n1 * L.n2;
+  const int64_t chunk = ((nint + (int64_t)gridDim.x * 256 - 1) / ((int64_t)gridDim.x * 256)) * 256;
+  const int64_t q0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk;
+  const int64_t q1 = q0 + chunk < nint ? q0 + chunk : nint;
+  double rr = 0.0;
+  for (int64_t q = q0 + threadIdx.x; q < q1; q += 256) {
+    const int c = (int)(q % L.n2) + 1, b = (int)((q / L.n2) % L.n1) + 1, a = (int)(q / ((int64_t)L.n1 * L.n2)) + 1;
+    const int64_t g = a * L.sx + b * L.sy + c;
+    const double ae = 6.0 * (double)e[g] - ((double)e[g - L.sx] + (double)e[g + L.sx] + (double)e[g - L.sy] +
+                                           (double)e[g + L.sy] + (double)e[g - 1] + (double)e[g + 1]);
+    const double t = (double)r[g] - ae;
+    rr += t * t;
+  }
+  s1[threadIdx.x] = rr;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s1[threadIdx.x] += s1[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// The pair's decision (one block, fixed tree): skip[0] = 1, so that every
+// kernel of the second V-cycle returns at once, when the solve is already
+// done or the first cycle's correction alone reaches the tolerance (against
+// ||b||^2 of the solve's start, dst[1]). The residual after the first cycle is
+// the cycles' right-hand side minus A e; that right-hand side is the FP64
+// residual rounded to P, whose rounding (~1e-7 of it in FP32) is far below
+// the ~30x a V-cycle removes, so the test agrees with the FP64 check that
+// follows the update except within rounding of the tolerance itself.
+__global__ __launch_bounds__(256) void k_pair_check(const double* part, int nb, const double* dst, int* st,
+                                                    double tol, int* skip) {
+  __shared__ double s1[256];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) a += part[2 * i];
+  s1[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s1[threadIdx.x] += s1[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double rr = s1[0], bb = dst[1];
+    const double rel = bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+    skip[0] = (st[0] || rel <= tol || (bb == 0.0 && rr < 1e-300)) ? 1 : 0;
+    if (!st[0]) st[6] += 1;  // (a pass that ran; k_pair_resid returns at once once done)
+  }
+}
+
 // Final sum of the residual partials (one block, fixed tree: deterministic),
 // cycle count and the stopping test of the solve loop. keep_bb: ||b||^2 is
-// the one already in dst[1] (the partials carry r^2 only).
+// the one already in dst[1] (the partials carry r^2 only). count_cycle V-cycles
+// were applied, less one when *skipped (a pair whose second cycle was skipped).
 __global__ __launch_bounds__(256) void k_norm(const double* part, int nb, double* dst, int* st, double tol,
-                                              int max_cycles, int count_cycle, int keep_bb) {
+                                              int max_cycles, int count_cycle, int keep_bb,
+                                              const int* skipped = nullptr) {
   if (st[0]) return;
   __shared__ double s1[256], s2[256];
   double a = 0.0, b = 0.0;
@@ -166,7 +233,8 @@ __global__ __launch_bounds__(256) void k_norm(const double* part, int nb, double
   }
   if (threadIdx.x == 0) {
     const double rr = s1[0], bb = keep_bb ? dst[1] : s2[0];
-    const int cyc = st[1] + count_cycle;
+    const int cyc = st[1] + count_cycle - (skipped ? skipped[0] : 0);
+    if (count_cycle > 0) st[5] += 1;  // an FP64 update preceded this check
     const double rel = bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
     dst[0] = rr;
     dst[1] = bb;
@@ -785,8 +853,12 @@ void smooth(Ctx& c, MgLevel& m, int sweeps, double w, bool from_zero, const int*
   }
 }
 
+// One V-cycle from level l down. zero: the level's correction starts from
+// e = 0 (always so below the finest level); otherwise the finest level's
+// cycle continues from the correction already in e (a second cycle on the
+// same right-hand side).
 template <class P>
-void vcycle(Ctx& c, int l, const int* done) {
+void vcycle(Ctx& c, int l, const int* done, bool zero = true) {
   MgLevel& m = c.mg[l];
   const double w = c.d.mg_omega;
   if (l == c.mg_tail) {
@@ -808,8 +880,8 @@ void vcycle(Ctx& c, int l, const int* done) {
   }
   MgLevel& mc = c.mg[l + 1];
   // every level starts its cycle from e = 0 (the correction equation)
-  if (c.d.mg_pre == 0) k_zero_level<P><<<grid_full(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), done);
-  smooth<P>(c, m, c.d.mg_pre, w, true, done);
+  if (zero && c.d.mg_pre == 0) k_zero_level<P><<<grid_full(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), done);
+  smooth<P>(c, m, c.d.mg_pre, w, zero, done);
   if ((int64_t)m.n[0] * m.n[1] * m.n[2] >= kSplitRestrict) {
     k_resid_level<P><<<grid_full4(m), 64, 0, c.stream>>>(T_<P>(m), E_<P>(m), R_<P>(m), lv_of(m), done);
     k_restrict_t<P><<<grid_full(mc), 64, 0, c.stream>>>(R_<P>(mc), lv_of(mc), T_<P>(m), lv_of(m), done);
@@ -855,51 +927,82 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, part, done, r64);
   k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, 0, 0);
   k_keep_start<<<1, 1, 0, c.stream>>>(dst);
-  // queue as many cycles as the previous solves needed, then two at a time
-  int batch = std::max(1, std::min(c.mg_hist[0], c.mg_hist[1]));
-  int32_t hs[2];
+  // Queue as many cycles as the previous two solves needed, then more. With
+  // pairs, two V-cycles share one FP64 update and norm check: the second
+  // runs on the same right-hand side, continuing from the first's correction
+  // (the residual it restricts is then r - A e, formed in P, whose rounding
+  // stays far below the ~30x a V-cycle removes). Between the two, one read of
+  // e and r gives the residual the first cycle left (k_pair_resid); when that
+  // already meets the tolerance the second cycle's kernels all return at
+  // once, so a solve applies exactly the cycles it needs, as with single
+  // cycles. A pair saves the FP64 update (reads and writes phi and r in FP64)
+  // and a norm check for the price of that one read.
+  const int pred = std::max(1, std::min(c.mg_hist[0], c.mg_hist[1]));
+  // (a finest level inside the single-workgroup tail, or the coarsest level
+  // itself, always solves from zero: no pairs there)
+  const bool pairs = c.mg_pairs && c.mg_tail > 0 && (int)c.mg.size() > 1;
+  int npair = pairs ? pred / 2 : 0;
+  int batch = pred - 2 * npair;
+  int32_t hs[7];
   double hd[4];
-  // One cycle: the V-cycle, the FP64 update and the norm check. Its ~30
-  // launches are captured once into a graph of kernel nodes (every pointer
-  // and size in them is fixed for the context; the tolerance and the cycle
-  // cap are arguments of the norm kernel, so the graph is rebuilt when they
-  // change) and replayed per cycle.
-  auto cycle = [&]() {
+  int* skip = st + 4;
+  // One cycle (or pair): the V-cycle(s), the FP64 update and the norm
+  // check. Their ~30 (~60) launches are captured once into a graph of kernel
+  // nodes (every pointer and size in them is fixed for the context; the
+  // tolerance and the cycle cap are arguments of the norm kernel, so the
+  // graphs are rebuilt when they change) and replayed.
+  auto cycle = [&](int nv) {
     vcycle<P>(c, 0, done);
-    k_update<P><<<nb, 256, 0, c.stream>>>(phi, E_<P>(m0), r64, r0, L0, part, done);
-    k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, 1, 1);
-  };
-  if (c.mg_use_graph && (!c.mg_graph || c.mg_graph_esz != sizeof(P) || c.mg_graph_tol != tol ||
-                         c.mg_graph_cap != max_cycles)) {
-    if (c.mg_graph) {
-      FOPIS_HIP(hipGraphExecDestroy(c.mg_graph));
-      c.mg_graph = nullptr;
+    if (nv == 2) {
+      k_pair_resid<P><<<nb, 256, 0, c.stream>>>(E_<P>(m0), r0, L0, part, done);
+      k_pair_check<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, skip);
+      vcycle<P>(c, 0, skip, false);
     }
-    hipGraph_t g = nullptr;
-    FOPIS_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    cycle();
-    const hipError_t ec = hipStreamEndCapture(c.stream, &g);
-    if (ec != hipSuccess || !g) throw Error(FOPIS_E_DEVICE, std::string("V-cycle capture: ") + hipGetErrorString(ec));
-    const hipError_t ei = hipGraphInstantiate(&c.mg_graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ei != hipSuccess) {
-      c.mg_graph = nullptr;
-      throw Error(FOPIS_E_DEVICE, std::string("V-cycle graph: ") + hipGetErrorString(ei));
+    k_update<P><<<nb, 256, 0, c.stream>>>(phi, E_<P>(m0), r64, r0, L0, part, done);
+    k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, nv, 1, nv == 2 ? skip : nullptr);
+  };
+  if (c.mg_use_graph && (!c.mg_graph[0] || c.mg_graph_esz != sizeof(P) || c.mg_graph_tol != tol ||
+                         c.mg_graph_cap != max_cycles)) {
+    for (int k = 0; k < 2; ++k) {
+      if (c.mg_graph[k]) {
+        FOPIS_HIP(hipGraphExecDestroy(c.mg_graph[k]));
+        c.mg_graph[k] = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      FOPIS_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      cycle(k + 1);
+      const hipError_t ec = hipStreamEndCapture(c.stream, &g);
+      if (ec != hipSuccess || !g)
+        throw Error(FOPIS_E_DEVICE, std::string("V-cycle capture: ") + hipGetErrorString(ec));
+      const hipError_t ei = hipGraphInstantiate(&c.mg_graph[k], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) {
+        c.mg_graph[k] = nullptr;
+        throw Error(FOPIS_E_DEVICE, std::string("V-cycle graph: ") + hipGetErrorString(ei));
+      }
     }
     c.mg_graph_esz = sizeof(P);
     c.mg_graph_tol = tol;
     c.mg_graph_cap = max_cycles;
   }
+  auto launch = [&](int nv) {
+    if (c.mg_use_graph)
+      FOPIS_HIP(hipGraphLaunch(c.mg_graph[nv - 1], c.stream));
+    else
+      cycle(nv);
+  };
   for (;;) {
-    for (int q = 0; q < batch; ++q) {
-      if (c.mg_use_graph)
-        FOPIS_HIP(hipGraphLaunch(c.mg_graph, c.stream));
-      else
-        cycle();
+    for (int q = 0; q < npair; ++q) launch(2);
+    for (int q = 0; q < batch; ++q) launch(1);
+    if (pairs) {  // later rounds: one pair (up to two cycles) at a time
+      npair = 1;
+      batch = 0;
+    } else {
+      npair = 0;
+      batch = 2;
     }
     FOPIS_HIP(hipMemcpyAsync(hs, st, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     FOPIS_HIP(hipStreamSynchronize(c.stream));
-    batch = 2;
     if (!hs[0]) continue;
     // confirm on the true residual b - A phi before stopping
     k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, part, never, r64);
@@ -911,6 +1014,8 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
     k_reopen<<<1, 1, 0, c.stream>>>(st);
   }
   const int cycles = hs[1];
+  c.mg_updates_last = hs[5];
+  c.mg_pair_passes_last = hs[6];
   c.mg_hist[1] = c.mg_hist[0];
   c.mg_hist[0] = cycles;
   // face conditions in the reference's order: x low/up, y, z
@@ -983,11 +1088,13 @@ void poisson_init(Ctx& c) {
     c.mg_r64.alloc(c.nnode);
     FOPIS_HIP(hipMemsetAsync(c.mg_r64.p, 0, c.nnode * sizeof(double), c.stream));
   }
-  c.mg_state.alloc(4);
-  FOPIS_HIP(hipMemsetAsync(c.mg_state.p, 0, 4 * sizeof(int32_t), c.stream));
+  c.mg_state.alloc(8);
+  FOPIS_HIP(hipMemsetAsync(c.mg_state.p, 0, 8 * sizeof(int32_t), c.stream));
   c.mg_part.alloc(2 * kResidBlocks + 4);
   const char* g = std::getenv("FOPIS_MG_GRAPH");
   c.mg_use_graph = !(g && g[0] == '0');
+  const char* pr2 = std::getenv("FOPIS_MG_PAIRS");
+  c.mg_pairs = !(pr2 && pr2[0] == '0');
   const char* cy = std::getenv("FOPIS_CARRY");
   c.carry = !(cy && cy[0] == '0');
   int dev = 0;
@@ -1001,8 +1108,10 @@ void poisson_init(Ctx& c) {
 }
 
 void poisson_free(Ctx& c) {
-  if (c.mg_graph) (void)hipGraphExecDestroy(c.mg_graph);
-  c.mg_graph = nullptr;
+  for (auto& g : c.mg_graph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
 }
 
 int poisson_solve(Ctx& c, double tol, int max_cycles, double* relres, const double* rhs_override) {
@@ -1016,6 +1125,8 @@ int poisson_solve(Ctx& c, double tol, int max_cycles, double* relres, const doub
   c.mg_cycles_last = cyc;
   c.mg_cycles_total += cyc;
   c.mg_solves_total += 1;
+  c.mg_updates_total += c.mg_updates_last;
+  c.mg_pair_passes_total += c.mg_pair_passes_last;
   c.relres_last = *relres;
   return cyc;
 }

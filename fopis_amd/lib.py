@@ -32,7 +32,7 @@ EXPORTS = [
     "fopis_guess_dataset_get", "fopis_guess_dataset_set", "fopis_guess_grad", "fopis_guess_train",
     "fopis_guess_eval", "fopis_guess_synth_boxes", "fopis_nn_conv3d", "fopis_nn_conv3d_backward",
     "fopis_first_solve_cycles", "fopis_moments", "fopis_guess_set_loss", "fopis_inject_maxwellian",
-    "fopis_emit_photoelectrons", "fopis_particle_ledger",
+    "fopis_emit_photoelectrons", "fopis_particle_ledger", "fopis_solver_passes",
 ]
 
 
@@ -360,6 +360,9 @@ def solver_stats(ctx: "Context") -> dict:
     ctx.L.fopis_solver_start_residuals.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     r0 = (C.c_double * 3)()
     ctx._ok(ctx.L.fopis_solver_start_residuals(ctx.h, r0))
+    ctx.L.fopis_solver_passes.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    pp = (C.c_int64 * 2)()
+    ctx._ok(ctx.L.fopis_solver_passes(ctx.h, pp))
     return {"last_cycles": lc.value, "last_relres": lr.value, "total_cycles": tc.value, "total_solves": ts.value,
             "first_cycles": fc.value, "start_relres_last": r0[0], "start_relres_first": r0[1],
-            "start_relres_second": r0[2]}
+            "start_relres_second": r0[2], "updates": pp[0], "pair_passes": pp[1]}

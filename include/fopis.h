@@ -283,6 +283,13 @@ int fopis_moments(fopis_ctx* ctx, int s, double* out);
  * step since fopis_reset_timers (the solve the learned guess starts; phase
  * "poisson_first" of fopis_phase_time times guess + that solve). */
 int fopis_first_solve_cycles(fopis_ctx* ctx, int64_t* total_cycles);
+/* Full-grid passes of the Poisson solves since fopis_reset_timers beyond the
+ * V-cycles themselves: out[0] FP64 defect updates (phi += e, r -= A e; one
+ * per cycle, or one per pair of cycles that share a right-hand side), out[1]
+ * residual passes between the two cycles of a pair (the bytes a solve
+ * moves, for its roofline). The reference's SOR has no such split
+ * (RootGridField.cpp:65-103). */
+int fopis_solver_passes(fopis_ctx* ctx, int64_t* out);
 /* Particle ledger of species s since the context was created: out[0]
  * staged particles (added, injected or emitted) that joined the particle
  * set inside the domain, out[1] particles absorbed at a domain face by a

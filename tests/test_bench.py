@@ -33,3 +33,8 @@ def test_poisson_roofline_bytes():
     assert r["frac"] == r["achieved"] / bench.HBM_PEAK_GBS
     D.mg_precision = 0
     assert bench.poisson_roofline(D, 4.0, 1.0)["algorithmic_bytes_per_solve"] == n * (166 + 40 + 56)
+    # pairs of cycles sharing one FP64 update, with a residual pass between the two
+    D.mg_precision = 1
+    r = bench.poisson_roofline(D, 3.0, 4.0, updates=2.0, pair_passes=2.0)
+    assert r["algorithmic_bytes_per_solve"] == n * (4 * 83 + 2 * 40 + 2 * 8 + 56)
+    assert r["updates_per_solve"] == 2.0 and r["pair_passes_per_solve"] == 2.0
