@@ -435,10 +435,10 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   __shared__ int sh_rank[4 * kWaves];
   __shared__ int s_res[2];  // DB: slots reserved so far in the region and in the outbox
   // where the binning pass stores a particle of each kind (0 tile region, 1
-  // outbox, 2 stray list, 3 sink): the set's base address, its component
-  // stride and the base of the direction byte, all in bytes; one LDS read per
-  // particle replaces a four-way choice among twelve 64-bit scalars
-  __shared__ __attribute__((aligned(16))) unsigned long long s_dst[(MODE & M_BIN) != 0 ? 4 : 1][4];
+  // outbox, 2 stray list, 3 sink): the set's base address and its component
+  // stride in bytes; one 16-byte LDS read per particle replaces a four-way
+  // choice among eight 64-bit scalars
+  __shared__ __attribute__((aligned(16))) unsigned long long s_dst[(MODE & M_BIN) != 0 ? 4 : 1][2];
   R* sE = reinterpret_cast<R*>(smem);
   unsigned long long* sR = reinterpret_cast<unsigned long long*>(smem + SM::e_bytes);
   // the count window is its own LDS array so that its atomics stay LDS atomics (ds_add_u32)
@@ -484,8 +484,6 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
       const uint64_t snk = reinterpret_cast<uint64_t>(a.sink);
       s_dst[k][0] = k == 3 ? snk : reinterpret_cast<uint64_t>(set.c[0]);
       s_dst[k][1] = k == 3 ? 0ull : reinterpret_cast<uint64_t>(set.c[1]) - reinterpret_cast<uint64_t>(set.c[0]);
-      s_dst[k][2] = k == 1 ? reinterpret_cast<uint64_t>(a.odir) : snk;
-      s_dst[k][3] = 0ull;
     }
   }
   __syncthreads();
@@ -511,6 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   const int64_t ost = ((MODE & M_BIN) != 0) ? a.ostart[t] : 0;
   const int rcap = a.capa[t];
   const int64_t sink_at = ((int64_t)t * kBlock + threadIdx.x) & (kSink - 1);
+  const uint64_t odir_b = reinterpret_cast<uint64_t>(a.odir), snk_b = reinterpret_cast<uint64_t>(a.sink);
 
   // PPT particles per thread and iteration (FP32: two chunks of kBlock, whose
   // loads are in flight together; one block scan ranks both), the next
@@ -577,9 +576,14 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           float e[3];
           if (li >= 0 && li < kTile && lj >= 0 && lj < kTile && lk >= 0 && lk < kTile) {
             const int b0 = (li * kEW + lj) * kEW + lk;
-            const float4* E4 = reinterpret_cast<const float4*>(sE) + b0;
-            const float4 n1 = E4[0], n2 = E4[kEW * kEW], n3 = E4[kEW], n4 = E4[kEW * kEW + kEW];
-            const float4 n5 = E4[1], n6 = E4[kEW * kEW + 1], n7 = E4[kEW + 1], n8 = E4[kEW * kEW + kEW + 1];
+            // volatile keeps each corner one 16-byte read (ds_read_b128, pad
+            // included) rather than the 12 bytes used (ds_read_b96), which the
+            // LDS serves more slowly on gfx950: 4% of this kernel at C3
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const volatile __attribute__((address_space(3))) f4v* E4 =
+                (const volatile __attribute__((address_space(3))) f4v*)sE + b0;
+            const f4v n1 = E4[0], n2 = E4[kEW * kEW], n3 = E4[kEW], n4 = E4[kEW * kEW + kEW];
+            const f4v n5 = E4[1], n6 = E4[kEW * kEW + 1], n7 = E4[kEW + 1], n8 = E4[kEW * kEW + kEW + 1];
             e[0] = __builtin_fmaf(v8, n8.x, __builtin_fmaf(v7, n7.x, __builtin_fmaf(v6, n6.x, __builtin_fmaf(v5, n5.x,
                    __builtin_fmaf(v4, n4.x, __builtin_fmaf(v3, n3.x, __builtin_fmaf(v2, n2.x, v1 * n1.x)))))));
             e[1] = __builtin_fmaf(v8, n8.y, __builtin_fmaf(v7, n7.y, __builtin_fmaf(v6, n6.y, __builtin_fmaf(v5, n5.y,
@@ -912,14 +916,15 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         // one base address and one component stride per destination: each
         // particle set is one allocation with its components a fixed stride
         // apart, and the sink takes all six components in one slot
-        const uint64_t db = s_dst[kind][0], ds = s_dst[kind][1], dd = s_dst[kind][2];
+        const uint64_t db = s_dst[kind][0], ds = s_dst[kind][1];
         uint64_t ad = db + (uint64_t)at * sizeof(R);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
           *reinterpret_cast<__attribute__((address_space(1))) R*>(ad) = p[u][c];
           ad += ds;
         }
-        const uint64_t od = dd + (uint64_t)(kind == 1 ? at : sink_at);
+        // the direction byte: the outbox's for a mover, the sink's otherwise
+        const uint64_t od = kind == 1 ? odir_b + (uint64_t)at : snk_b + (uint64_t)sink_at;
         *reinterpret_cast<__attribute__((address_space(1))) uint8_t*>(od) = (uint8_t)dir[u];
         wb += ts[u];
         ob0 += tm[u];

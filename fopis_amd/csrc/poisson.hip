@@ -52,33 +52,48 @@ __device__ inline void block3(int& bx, int& by, int& bz) {
   bz = L / (gx * gy);
 }
 
+// The level-0 passes below (residuals, update) walk the interior by rows:
+// each workgroup takes one contiguous range of (a, b) rows (XCD-renumbered, so
+// that neighbouring planes stay in one XCD's L2) and spreads its 256 threads
+// over 1, 2 or 4 rows at a time, a row's points over 256, 128 or 64 threads.
+// f(g) gets the node index of every interior point once.
+template <class F>
+__device__ inline void for_interior_rows(const int n0, const int n1, const int n2, const int64_t sx,
+                                         const int64_t sy, F&& f) {
+  const int nrows = n0 * n1;
+  const int per = (nrows + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int r0 = xcd_block(blockIdx.x, gridDim.x) * per;
+  const int r1 = min(r0 + per, nrows);
+  const int tpr = n2 <= 64 ? 64 : (n2 <= 128 ? 128 : 256);  // threads per row
+  const int sub = threadIdx.x / tpr, lane = threadIdx.x % tpr;
+  for (int row = r0 + sub; row < r1; row += 256 / tpr) {
+    const int a = row / n1 + 1, b = row % n1 + 1;
+    const int64_t base = a * sx + b * sy;
+    for (int c = lane + 1; c <= n2; c += tpr) f(base + c);
+  }
+}
+
 // b = rc * rho at the level-0 interior; r = b - A phi written as P;
 // per-block partial sums of r^2 and b^2 in FP64.
-// (r64, when given, receives the FP64 residual that the cycle updates keep.)
+// (r64, when given, receives the FP64 residual that the cycle updates keep;
+// with r and r64 both null only the norms are formed: the check at a solve's end.)
 template <class P>
 __global__ __launch_bounds__(256) void k_resid0(const double* __restrict__ phi, const double* __restrict__ rho,
                                                 double rc, P* __restrict__ r, Lv L, double* part,
                                                 const int* __restrict__ done, double* __restrict__ r64) {
   if (*done) return;
   __shared__ double s1[256], s2[256];
-  const int64_t nint = (int64_t)L.n0 * L.n1 * L.n2;
-  // each workgroup sums one contiguous range of points (XCD-renumbered)
-  const int64_t chunk = ((nint + (int64_t)gridDim.x * 256 - 1) / ((int64_t)gridDim.x * 256)) * 256;
-  const int64_t q0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk;
-  const int64_t q1 = q0 + chunk < nint ? q0 + chunk : nint;
   double rr = 0.0, bb = 0.0;
-  for (int64_t q = q0 + threadIdx.x; q < q1; q += 256) {
-    const int c = (int)(q % L.n2) + 1, b = (int)((q / L.n2) % L.n1) + 1, a = (int)(q / ((int64_t)L.n1 * L.n2)) + 1;
-    const int64_t g = a * L.sx + b * L.sy + c;
+  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g) {
     const double bv = rc * rho[g];
     const double ap = 6.0 * phi[g] - (phi[g - L.sx] + phi[g + L.sx] + phi[g - L.sy] + phi[g + L.sy] + phi[g - 1] +
                                       phi[g + 1]);
     const double rv = bv - ap;
-    r[g] = (P)rv;
+    if (r) r[g] = (P)rv;
     if (r64) r64[g] = rv;
     rr += rv * rv;
     bb += bv * bv;
-  }
+  });
   s1[threadIdx.x] = rr;
   s2[threadIdx.x] = bb;
   __syncthreads();
@@ -117,15 +132,9 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ phi, const 
                                                 P* rp, Lv L, double* part, const int* __restrict__ done) {
   if (*done) return;
   __shared__ double s1[256];
-  const int64_t nint = (int64_t)L.n0 * L.n1 * L.n2;
-  const int64_t chunk = ((nint + (int64_t)gridDim.x * 256 - 1) / ((int64_t)gridDim.x * 256)) * 256;
-  const int64_t q0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk;
-  const int64_t q1 = q0 + chunk < nint ? q0 + chunk : nint;
   const bool same = (const void*)rp == (const void*)r64;
   double rr = 0.0;
-  for (int64_t q = q0 + threadIdx.x; q < q1; q += 256) {
-    const int c = (int)(q % L.n2) + 1, b = (int)((q / L.n2) % L.n1) + 1, a = (int)(q / ((int64_t)L.n1 * L.n2)) + 1;
-    const int64_t g = a * L.sx + b * L.sy + c;
+  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g) {
     const double eg = (double)e[g];
     const double ae = 6.0 * eg - ((double)e[g - L.sx] + (double)e[g + L.sx] + (double)e[g - L.sy] +
                                   (double)e[g + L.sy] + (double)e[g - 1] + (double)e[g + 1]);
@@ -134,7 +143,7 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ phi, const 
     if (!same) rp[g] = (P)rv;
     phi[g] += eg;
     rr += rv * rv;
-  }
+  });
   s1[threadIdx.x] = rr;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -149,25 +158,18 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ phi, const 
 
 // Between the two V-cycles of a pair: per-block partial sums of t^2, t = r - A e
 // at the level-0 interior (the cycles' right-hand side and the first cycle's
-// correction, summed in FP64; the b^2 slot is 0).
+// correction; t formed in P as the cycles form it, its squares summed in
+// FP64; the b^2 slot is 0).
 template <class P>
 __global__ __launch_bounds__(256) void k_pair_resid(const P* __restrict__ e, const P* __restrict__ r, Lv L,
                                                     double* part, const int* __restrict__ done) {
   if (*done) return;
   __shared__ double s1[256];
-  const int64_t nint = (int64_t)L.n0 * L.n1 * L.n2;
-  const int64_t chunk = ((nint + (int64_t)gridDim.x * 256 - 1) / ((int64_t)gridDim.x * 256)) * 256;
-  const int64_t q0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * chunk;
-  const int64_t q1 = q0 + chunk < nint ? q0 + chunk : nint;
   double rr = 0.0;
-  for (int64_t q = q0 + threadIdx.x; q < q1; q += 256) {
-    const int c = (int)(q % L.n2) + 1, b = (int)((q / L.n2) % L.n1) + 1, a = (int)(q / ((int64_t)L.n1 * L.n2)) + 1;
-    const int64_t g = a * L.sx + b * L.sy + c;
-    const double ae = 6.0 * (double)e[g] - ((double)e[g - L.sx] + (double)e[g + L.sx] + (double)e[g - L.sy] +
-                                           (double)e[g + L.sy] + (double)e[g - 1] + (double)e[g + 1]);
-    const double t = (double)r[g] - ae;
-    rr += t * t;
-  }
+  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g) {
+    const P t = r[g] - ((P)6 * e[g] - (e[g - L.sx] + e[g + L.sx] + e[g - L.sy] + e[g + L.sy] + e[g - 1] + e[g + 1]));
+    rr += (double)t * (double)t;
+  });
   s1[threadIdx.x] = rr;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -1005,12 +1007,15 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
     FOPIS_HIP(hipStreamSynchronize(c.stream));
     if (!hs[0]) continue;
     // confirm on the true residual b - A phi before stopping
-    k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, part, never, r64);
+    k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, (P*)nullptr, L0, part, never, (double*)nullptr);
     k_norm_final<<<1, 256, 0, c.stream>>>(part, nb, dst);
     FOPIS_HIP(hipMemcpyAsync(hd, dst, sizeof(hd), hipMemcpyDeviceToHost, c.stream));
     FOPIS_HIP(hipStreamSynchronize(c.stream));
     const bool zero_rhs = hd[1] == 0.0 && hd[0] < 1e-300;
     if (hd[2] <= tol || zero_rhs || hs[1] >= max_cycles) break;
+    // the recursively updated residual has drifted from the true one: the
+    // further cycles start from the true residual
+    k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, part, never, r64);
     k_reopen<<<1, 1, 0, c.stream>>>(st);
   }
   const int cycles = hs[1];
