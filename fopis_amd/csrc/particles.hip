@@ -451,6 +451,22 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
   // per axis: the tile's origin minus one, and the cells from the origin to the grid's end
   const int om1[3] = {o0 - 1, o1 - 1, o2 - 1};
   const int nmo[3] = {a.N[0] - o0, a.N[1] - o1, a.N[2] - o2};
+  // FP32 particles are stored relative to their tile's origin (a coordinate
+  // then resolves 2^-21 of a cell or better across the tile, where an absolute
+  // one resolves 2^-16 near x = 255); FP64 particles keep absolute
+  // coordinates. Per axis, in stored coordinates: wo turns the floor of a
+  // coordinate into the cell counted from the tile's origin minus one, cl is
+  // the floor of the last cell, lo and hi bound the domain.
+  constexpr bool kLocal = sizeof(R) == 4;
+  int wo[3];
+  R cl[3], lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    wo[k] = kLocal ? -1 : om1[k];
+    cl[k] = (R)(kLocal ? nmo[k] - 1 : a.N[k] - 1);
+    lo[k] = kLocal ? (R)(-ok[k]) : (R)0;
+    hi[k] = kLocal ? a.rxmax[k] - (R)ok[k] : a.rxmax[k];
+  }
 
   if constexpr (SM::kE) {
     constexpr int ne = kEW * kEW * kEW;
@@ -536,7 +552,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     if (idx >= nreg) {
       if (idx < ntot) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) q[c] = (R)a.in[c * a.in_s + ib0 + idx - nreg];
+        for (int c = 0; c < 6; ++c)
+          q[c] = (R)(a.in[c * a.in_s + ib0 + idx - nreg] - (kLocal && c < 3 ? (double)ok[c] : 0.0));
       } else {
 #pragma unroll
         for (int c = 0; c < 6; ++c) q[c] = (R)0;
@@ -561,10 +578,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         // and factored weights (FP32 makes no claim of bit equality with the
         // reference, so the operation order is free)
         if (valid && !from_in) {
-          const float fx = fminf(floorf(p[0]), (float)(a.N[0] - 1));
-          const float fy = fminf(floorf(p[1]), (float)(a.N[1] - 1));
-          const float fz = fminf(floorf(p[2]), (float)(a.N[2] - 1));
-          const int ci = (int)fx, cj = (int)fy, ck = (int)fz;
+          const float fx = fminf(floorf(p[0]), cl[0]);
+          const float fy = fminf(floorf(p[1]), cl[1]);
+          const float fz = fminf(floorf(p[2]), cl[2]);
+          // the cell counted from the tile's origin (the coordinates are)
+          const int li = (int)fx, lj = (int)fy, lk = (int)fz;
           const float dx1 = p[0] - fx, dy1 = p[1] - fy, dz1 = p[2] - fz;
           const float dx2 = 1.0f - dx1, dy2 = 1.0f - dy1, dz2 = 1.0f - dz1;
           const float qm = a.rqm;
@@ -572,7 +590,6 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           const float a22 = dx2 * dy2, a12 = dx1 * dy2, a21 = dx2 * dy1, a11 = dx1 * dy1;
           const float v1 = a22 * qz2, v2 = a12 * qz2, v3 = a21 * qz2, v4 = a11 * qz2;
           const float v5 = a22 * qz1, v6 = a12 * qz1, v7 = a21 * qz1, v8 = a11 * qz1;
-          const int li = ci - o0, lj = cj - o1, lk = ck - o2;
           float e[3];
           if (li >= 0 && li < kTile && lj >= 0 && lj < kTile && lk >= 0 && lk < kTile) {
             const int b0 = (li * kEW + lj) * kEW + lk;
@@ -591,6 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
             e[2] = __builtin_fmaf(v8, n8.z, __builtin_fmaf(v7, n7.z, __builtin_fmaf(v6, n6.z, __builtin_fmaf(v5, n5.z,
                    __builtin_fmaf(v4, n4.z, __builtin_fmaf(v3, n3.z, __builtin_fmaf(v2, n2.z, v1 * n1.z)))))));
           } else {  // outside the tile (a bin left stale by an interrupted step): global gather
+            const int ci = li + o0, cj = lj + o1, ck = lk + o2;
             const size_t sI = (size_t)a.cy * a.cz, sJ = a.cz;
             const size_t g = (size_t)(ci + 1) * sI + (size_t)(cj + 1) * sJ + (ck + 1);
 #pragma unroll
@@ -669,8 +687,8 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
         }
       }
       // outside the domain (a pushed particle is absorbed there, a staged one is dropped)
-      const bool out = (p[0] < (R)0) | (p[0] > a.rxmax[0]) | (p[1] < (R)0) | (p[1] > a.rxmax[1]) |
-                       (p[2] < (R)0) | (p[2] > a.rxmax[2]);
+      const bool out = (p[0] < lo[0]) | (p[0] > hi[0]) | (p[1] < lo[1]) | (p[1] > hi[1]) | (p[2] < lo[2]) |
+                       (p[2] > hi[2]);
       if constexpr ((MODE & M_POS) != 0) {
         lost = valid & !from_in & out;
         valid = valid & !out;
@@ -690,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         fl[k] = floor(p[k]);
-        w[k] = (int)fl[k] - om1[k];
+        w[k] = (int)fl[k] - wo[k];
         wd[k] = min(w[k], nmo[k]);
       }
       const bool near = max(max((unsigned)w[0], (unsigned)w[1]), (unsigned)w[2]) < (unsigned)kDW;
@@ -727,7 +745,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
             if (slot < a.cap_cap) {
               double* r = a.cap_rec + (size_t)8 * slot;
 #pragma unroll
-              for (int c = 0; c < 6; ++c) r[c] = (double)p[c];
+              for (int c = 0; c < 6; ++c) r[c] = (double)p[c] + (kLocal && c < 3 ? (double)ok[c] : 0.0);
               r[6] = (double)(m - 1);
             } else {
               atomicAdd(&a.status[6], 1);
@@ -736,8 +754,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           }
         }
         if (valid) {
-          const R dx1 = p[0] - fmin(fl[0], (R)(a.N[0] - 1)), dy1 = p[1] - fmin(fl[1], (R)(a.N[1] - 1)),
-                  dz1 = p[2] - fmin(fl[2], (R)(a.N[2] - 1));
+          const R dx1 = p[0] - fmin(fl[0], cl[0]), dy1 = p[1] - fmin(fl[1], cl[1]), dz1 = p[2] - fmin(fl[2], cl[2]);
           const R dx2 = (R)1 - dx1, dy2 = (R)1 - dy1, dz2 = (R)1 - dz1;
           // CIC weights of the eight nodes (the particle's charge is applied when
           // the window is written out)
@@ -909,6 +926,10 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
           if (slot < a.stray_cap) {
             kind = 2;
             at = slot;
+            if constexpr (kLocal) {  // the stray list holds absolute coordinates
+#pragma unroll
+              for (int k = 0; k < 3; ++k) p[u][k] += (R)ok[k];
+            }
           } else {
             atomicAdd(&a.status[1], 1);
           }
@@ -1000,12 +1021,24 @@ __global__ __launch_bounds__(kBlock) void k_scatter(TileArgs<R> a, int32_t* __re
       R p[6];
 #pragma unroll
       for (int c = 0; c < 6; ++c) p[c] = a.out.c[c][ob + i];
+      if constexpr (sizeof(R) == 4) {
+        // FP32 outbox entries are relative to the source tile's origin; the
+        // destination's origin lies one tile width away per step of d
+        p[0] += (R)(kTile * (1 - d / 9));
+        p[1] += (R)(kTile * (1 - (d / 3) % 3));
+        p[2] += (R)(kTile * (1 - d % 3));
+      }
       const int at = a.cnt[t] + pos;
       if (at < a.capa[t]) {
         const int64_t g = a.start[t] + at;
 #pragma unroll
         for (int c = 0; c < 6; ++c) a.dst.c[c][g] = p[c];
       } else {
+        if constexpr (sizeof(R) == 4) {  // the stray list holds absolute coordinates
+          p[0] += (R)(kTile * (sx + d / 9 - 1));
+          p[1] += (R)(kTile * (sy + (d / 3) % 3 - 1));
+          p[2] += (R)(kTile * (sz + d % 3 - 1));
+        }
         put_stray(a, p);
       }
     }
@@ -1098,37 +1131,55 @@ __global__ void k_capacity(const uint32_t* cnt, int nt, double slack, int minc, 
   ocapa[t] = oc;
 }
 
+// The tile's origin (cells) in the coordinates of its particles' storage:
+// FP32 tile regions hold coordinates relative to it.
+__device__ __forceinline__ void tile_origin(uint32_t t, int t1, int t2, int o[3]) {
+  o[2] = (int)(t % (uint32_t)t2) * kTile;
+  o[1] = (int)((t / (uint32_t)t2) % (uint32_t)t1) * kTile;
+  o[0] = (int)(t / ((uint32_t)t2 * (uint32_t)t1)) * kTile;
+}
+
+// Particles of a flat set (absolute coordinates) into their tile regions;
+// t1, t2: tiles along y and z (FP32 regions are relative to the tile's origin).
 template <class R>
 __global__ void k_fill_regions(SoaPtr<R> dst, const int64_t* start, SoaPtr<R> src, const uint32_t* keys,
-                               const uint32_t* perm, int64_t n, int shift, const int64_t* first, uint32_t ntiles) {
+                               const uint32_t* perm, int64_t n, int shift, const int64_t* first, uint32_t ntiles,
+                               int t1, int t2) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t t = keys[i] >> shift;
   if (t >= ntiles) return;
   const int64_t pos = start[t] + (i - first[t]);
   const uint32_t j = perm[i];
+  int o[3] = {0, 0, 0};
+  if (sizeof(R) == 4 && t1 > 0) tile_origin(t, t1, t2, o);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) dst.c[c][pos] = src.c[c][j];
+  for (int c = 0; c < 6; ++c) dst.c[c][pos] = src.c[c][j] - (c < 3 ? (R)o[c] : (R)0);
 }
 
-template <class R>
-__global__ __launch_bounds__(kBlock) void k_gather_regions(SoaPtr<R> dst, const int64_t* first, SoaPtr<R> src,
-                                                           const int64_t* start, const int32_t* cnt) {
+// Tile regions into one flat set in tile order, in absolute coordinates of
+// type O (the origin of an FP32 region added in O: exact into FP64).
+template <class R, class O>
+__global__ __launch_bounds__(kBlock) void k_gather_regions(SoaPtr<O> dst, const int64_t* first, SoaPtr<R> src,
+                                                           const int64_t* start, const int32_t* cnt, int t1,
+                                                           int t2) {
   const int t = blockIdx.x;
   const int64_t s0 = start[t], d0 = first[t];
   const int n = cnt[t];
+  int o[3] = {0, 0, 0};
+  if (sizeof(R) == 4) tile_origin((uint32_t)t, t1, t2, o);
   for (int i = threadIdx.x; i < n; i += kBlock) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) dst.c[c][d0 + i] = src.c[c][s0 + i];
+    for (int c = 0; c < 6; ++c) dst.c[c][d0 + i] = (O)src.c[c][s0 + i] + (c < 3 ? (O)o[c] : (O)0);
   }
 }
 
-template <class R>
-__global__ void k_copy_soa(SoaPtr<R> dst, int64_t doff, SoaPtr<R> src, int64_t n) {
+template <class R, class O>
+__global__ void k_copy_soa(SoaPtr<O> dst, int64_t doff, SoaPtr<R> src, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) dst.c[c][doff + i] = src.c[c][i];
+  for (int c = 0; c < 6; ++c) dst.c[c][doff + i] = (O)src.c[c][i];
 }
 
 template <class R>
@@ -1643,15 +1694,15 @@ void bin_flat(Ctx& c, int s, Soa<R>& flat, int64_t n, bool key_cell) {
   if (n > 0) {
     SoaPtr<R> dst = ptrs(b.reg);
     k_fill_regions<R><<<div_up(n, 256), 256, 0, st>>>(dst, b.start.p, ptrs(flat), sc.keys.p, sc.perm.p, n, shift,
-                                                      sc.first.p, (uint32_t)nt);
+                                                      sc.first.p, (uint32_t)nt, c.nt[1], c.nt[2]);
   }
   FOPIS_HIP(hipGetLastError());
 }
 
 // Copies every live particle of species s into `flat` (tile order, then the
 // strays) and returns the count.
-template <class R>
-int64_t gather_all(Ctx& c, int s, Soa<R>& flat, bool with_strays) {
+template <class R, class O = R>
+int64_t gather_all(Ctx& c, int s, Soa<O>& flat, bool with_strays) {
   auto& b = bins<R>(c)[s];
   Scratch& sc = scratch(c);
   hipStream_t st = c.stream;
@@ -1671,8 +1722,10 @@ int64_t gather_all(Ctx& c, int s, Soa<R>& flat, bool with_strays) {
   const int64_t nstray = with_strays ? std::min<int64_t>(stat[0], b.stray_cap) : 0;
   const int64_t n = nreg + nstray;
   if ((int64_t)flat.size() < n) flat.alloc(n);
-  if (nreg > 0) k_gather_regions<R><<<nt, kBlock, 0, st>>>(ptrs(flat), sc.first.p, ptrs(b.reg), b.start.p, b.cnt.p);
-  if (nstray > 0) k_copy_soa<R><<<div_up(nstray, 256), 256, 0, st>>>(ptrs(flat), nreg, ptrs(b.stray), nstray);
+  if (nreg > 0)
+    k_gather_regions<R, O><<<nt, kBlock, 0, st>>>(ptrs(flat), sc.first.p, ptrs(b.reg), b.start.p, b.cnt.p, c.nt[1],
+                                                  c.nt[2]);
+  if (nstray > 0) k_copy_soa<R, O><<<div_up(nstray, 256), 256, 0, st>>>(ptrs(flat), nreg, ptrs(b.stray), nstray);
   FOPIS_HIP(hipGetLastError());
   return n;
 }
@@ -1731,7 +1784,7 @@ void prepare_inbox(Ctx& c, int s, bool consume) {
   b.inbox.reserve(n);
   // inbox in tile order (invalid ones sort past ibeg[nt] and are not copied)
   k_fill_regions<double><<<div_up(n, 256), 256, 0, st>>>(ptrs(b.inbox), sc.first.p, ptrs(raw), sc.keys.p,
-                                                         sc.perm.p, n, 0, sc.first.p, (uint32_t)nt);
+                                                         sc.perm.p, n, 0, sc.first.p, (uint32_t)nt, 0, 0);
   k_i32_from_i64<<<div_up(nt + 1, 256), 256, 0, st>>>(sc.first.p, b.ibeg.p, nt + 1);
   FOPIS_HIP(hipGetLastError());
   // the pinned staging may be refilled once the upload has been consumed by the
@@ -2009,8 +2062,9 @@ int64_t particles_count(Ctx& c, int s) { return c.fp32 ? count_impl<float>(c, s)
 template <class R>
 static void get_impl(Ctx& c, int s, int64_t cap, double* const* h, int64_t* nout) {
   auto& b = bins<R>(c)[s];
-  Soa<R> flat;
-  const int64_t n = gather_all<R>(c, s, flat, false);
+  // gathered straight into FP64 (an FP32 region's origin is added exactly)
+  Soa<double> out;
+  const int64_t n = gather_all<R, double>(c, s, out, false);
   // particles staged for the next step follow the resident ones: host-staged
   // first, then the ones drawn on the device (the order they are binned in);
   // staged particles outside the domain, which binning drops, are left out
@@ -2019,12 +2073,6 @@ static void get_impl(Ctx& c, int s, int64_t cap, double* const* h, int64_t* nout
   for (int q = 0; q < 6 && nd > 0; ++q)
     FOPIS_HIP(hipMemcpyAsync(dv.data() + q * (size_t)nd, b.dev_inbox.c[q].p, nd * sizeof(double),
                              hipMemcpyDeviceToHost, c.stream));
-  Soa<double> out;
-  if (n > 0) {
-    out.alloc(n);
-    k_to_f64<R><<<div_up(n, 256), 256, 0, c.stream>>>(ptrs(out), ptrs(flat), n);
-    FOPIS_HIP(hipGetLastError());
-  }
   FOPIS_HIP(hipStreamSynchronize(c.stream));
   const double lim[3] = {c.nx - 1.0, c.ny - 1.0, c.nz - 1.0};
   std::vector<int64_t> keep;
@@ -2352,8 +2400,13 @@ static void moments_impl(Ctx& c, int s, double* out) {
   out[0] = (double)n;
   if (n == 0) return;
   double sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int t = 0; t < nb; ++t)
+  for (int t = 0; t < nb; ++t) {
     for (int q = 0; q < 9; ++q) sum[q] += h[(size_t)t * 9 + q];
+    if (sizeof(R) == 4) {  // FP32 regions are relative to the tile's origin
+      const int o[3] = {t / (c.nt[2] * c.nt[1]) * kTile, (t / c.nt[2]) % c.nt[1] * kTile, t % c.nt[2] * kTile};
+      for (int q = 0; q < 3; ++q) sum[q] += (double)cnt[t] * o[q];
+    }
+  }
   for (int q = 0; q < 6; ++q) out[1 + q] = sum[q] / (double)n;
   for (int q = 0; q < 3; ++q) out[7 + q] = sum[6 + q] / (double)n - out[4 + q] * out[4 + q];
 }
