@@ -313,34 +313,50 @@ __global__ void k_smooth(P* __restrict__ e, const P* __restrict__ r, Lv L, int c
 
 // First full sweep of a cycle from e = 0, both colours in one pass: red
 // points get (w/6) r, black points the black half-sweep over those red values
-// (formed here from r). The same operations in the same order as a red
-// half-sweep from zero (red points (w/6) r, black points cleared) followed by
-// the black half-sweep, so the same result bit for bit, but r is read once and
-// e written once. Points outside the interior have r = 0 and so read as red
-// values of 0, as the untouched level borders do.
+// (formed here from r): the operations of a red half-sweep from zero (red
+// points (w/6) r, black points cleared) followed by the black half-sweep, in
+// the same order (the compiler's fused multiply-adds may round them
+// differently), but r is read once and e written once. Points outside the
+// interior have r = 0 and so read as red values of 0, as the untouched level
+// borders do.
 template <class P>
 __global__ void k_smooth_first2(P* __restrict__ e, const P* __restrict__ r, Lv L, P w1, P w6,
                                 const int* __restrict__ done) {
   if (*done) return;
   int bx, by, bz;
   block3(bx, by, bz);
-  const int c = bx * blockDim.x + threadIdx.x + 1;
+  // two neighbouring points of a row per thread, one of each colour, so the
+  // lanes of a wavefront all take the same path; every load of the kRows rows
+  // is issued before the first store (rows past the level read row 1 instead)
+  const int c0 = 2 * (bx * blockDim.x + threadIdx.x) + 1;
   const int a = bz + 1;
-  if (c > L.n2) return;
+  if (c0 > L.n2) return;
+  const bool two = c0 + 1 <= L.n2;
+  P vb[kRows], vr[kRows];
+  int64_t gbq[kRows], grq[kRows];
+  bool okq[kRows], fbq[kRows];
 #pragma unroll
   for (int q = 0; q < kRows; ++q) {
     const int b = by * kRows + q + 1;
-    if (b > L.n1) break;
-    const int64_t g = a * L.sx + b * L.sy + c;
-    P v;
-    if (((a + b + c) & 1) == 0) {
-      v = w6 * r[g];
-    } else {
-      const P nb = w6 * r[g - L.sx] + w6 * r[g + L.sx] + w6 * r[g - L.sy] + w6 * r[g + L.sy] + w6 * r[g - 1] +
-                   w6 * r[g + 1];
-      v = w1 * (P)0 + w6 * (nb + r[g]);
-    }
-    e[g] = v;
+    okq[q] = b <= L.n1;
+    const int bb = okq[q] ? b : 1;
+    const int64_t g0 = a * L.sx + bb * L.sy + c0;
+    // the black point of the pair is c0 when (a + b + c0) is odd, else c0 + 1
+    fbq[q] = ((a + bb + c0) & 1) != 0;
+    const int64_t gb = fbq[q] ? g0 : g0 + 1;
+    const int64_t gr = fbq[q] ? g0 + 1 : g0;
+    gbq[q] = gb;
+    grq[q] = gr;
+    const P nb = w6 * r[gb - L.sx] + w6 * r[gb + L.sx] + w6 * r[gb - L.sy] + w6 * r[gb + L.sy] + w6 * r[gb - 1] +
+                 w6 * r[gb + 1];
+    vb[q] = w1 * (P)0 + w6 * (nb + r[gb]);
+    vr[q] = w6 * r[gr];
+  }
+#pragma unroll
+  for (int q = 0; q < kRows; ++q) {
+    if (!okq[q]) continue;
+    if (fbq[q] || two) e[gbq[q]] = vb[q];
+    if (!fbq[q] || two) e[grq[q]] = vr[q];
   }
 }
 
@@ -847,7 +863,7 @@ void smooth(Ctx& c, MgLevel& m, int sweeps, double w, bool from_zero, const int*
   const P w1 = (P)(1.0 - w), w6 = (P)(w / 6.0);
   for (int s = 0; s < sweeps; ++s) {
     if (from_zero && s == 0) {
-      k_smooth_first2<P><<<grid_full4(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), w1, w6, done);
+      k_smooth_first2<P><<<grid_half4(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), w1, w6, done);
       continue;
     }
     for (int col = 0; col < 2; ++col)
