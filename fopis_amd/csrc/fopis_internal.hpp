@@ -309,6 +309,12 @@ struct Ctx {
   double mg_graph_tol = 0.0;  // tolerance and cycle cap captured in their norm kernels
   int mg_graph_cap = 0;
   bool mg_pairs = true;  // a solve's predicted cycles run two per FP64 update
+  // FP32 finest level stored by colour (red-black halves, see poisson.hip):
+  // e and r of colour k at erb/rrb + k * rb_cs, point (a, b, c) at
+  // a * rb_sx + b * rb_sy + c / 2
+  bool mg_rb = false;
+  DBuf<float> erb, rrb;
+  int64_t rb_cs = 0, rb_sx = 0, rb_sy = 0;
   bool mg_use_graph = true;   // replay the captured cycle (FOPIS_MG_GRAPH=0 launches it kernel by kernel)
   DBuf<double> mg_r64;        // FP64 level-0 residual when the V-cycle is FP32
   int mg_hist[2] = {1, 1};    // cycles of the last two solves (batch size of the next)

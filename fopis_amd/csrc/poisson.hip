@@ -56,7 +56,7 @@ __device__ inline void block3(int& bx, int& by, int& bz) {
 // each workgroup takes one contiguous range of (a, b) rows (XCD-renumbered, so
 // that neighbouring planes stay in one XCD's L2) and spreads its 256 threads
 // over 1, 2 or 4 rows at a time, a row's points over 256, 128 or 64 threads.
-// f(g) gets the node index of every interior point once.
+// f(g, a, b, c) gets the node index and the coordinates of every interior point once.
 template <class F>
 __device__ inline void for_interior_rows(const int n0, const int n1, const int n2, const int64_t sx,
                                          const int64_t sy, F&& f) {
@@ -69,7 +69,7 @@ __device__ inline void for_interior_rows(const int n0, const int n1, const int n
   for (int row = r0 + sub; row < r1; row += 256 / tpr) {
     const int a = row / n1 + 1, b = row % n1 + 1;
     const int64_t base = a * sx + b * sy;
-    for (int c = lane + 1; c <= n2; c += tpr) f(base + c);
+    for (int c = lane + 1; c <= n2; c += tpr) f(base + c, a, b, c);
   }
 }
 
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_resid0(const double* __restrict__ phi, 
   if (*done) return;
   __shared__ double s1[256], s2[256];
   double rr = 0.0, bb = 0.0;
-  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g) {
+  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g, int, int, int) {
     const double bv = rc * rho[g];
     const double ap = 6.0 * phi[g] - (phi[g - L.sx] + phi[g + L.sx] + phi[g - L.sy] + phi[g + L.sy] + phi[g - 1] +
                                       phi[g + 1]);
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_update(double* __restrict__ phi, const 
   __shared__ double s1[256];
   const bool same = (const void*)rp == (const void*)r64;
   double rr = 0.0;
-  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g) {
+  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g, int, int, int) {
     const double eg = (double)e[g];
     const double ae = 6.0 * eg - ((double)e[g - L.sx] + (double)e[g + L.sx] + (double)e[g - L.sy] +
                                   (double)e[g + L.sy] + (double)e[g - 1] + (double)e[g + 1]);
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_pair_resid(const P* __restrict__ e, con
   if (*done) return;
   __shared__ double s1[256];
   double rr = 0.0;
-  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g) {
+  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g, int, int, int) {
     const P t = r[g] - ((P)6 * e[g] - (e[g - L.sx] + e[g + L.sx] + e[g - L.sy] + e[g + L.sy] + e[g - 1] + e[g + 1]));
     rr += (double)t * (double)t;
   });
@@ -455,6 +455,282 @@ __global__ void k_prolong(P* __restrict__ e, Lv F, const P* __restrict__ ec, Lv 
                  E(a0, b1, c1) + E(a1, b1, c1)) *
                 (P)0.125;
     e[a * F.sx + b * F.sy + c] += v;
+  }
+}
+
+// ---- the finest FP32 level stored by colour ------------------------------
+//
+// A red-black half-sweep updates the points of one colour from those of the
+// other. With both colours in one array every cache line holds both, so a
+// half-sweep reads all of e and r and writes all of e: 201 MB per half-sweep
+// at 255^3 where the data it needs is 136 MB. Stored by colour, it reads the
+// other colour's e and its own e and r and writes its own e, each half-array
+// once. Point (a, b, c) of the padded level (interior 1..n) has colour
+// (a + b + c) & 1 and sits at a * sx + b * sy + c / 2 of its colour's array
+// (rows of (n2 + 3) / 2); its x neighbours c - 1 and c + 1 are at (c - 1) / 2
+// and (c + 1) / 2 of the other colour's row, its y and z neighbours at c / 2
+// of the other colour's neighbouring rows. Every kernel here forms the same
+// sums in the same order as its one-array counterpart, so a solve gives the
+// same result bit for bit either way.
+struct Rb {
+  float* e;  // colour 0; colour 1 at + cs
+  float* r;
+  int64_t cs, sx, sy;
+  int n0, n1, n2;
+};
+
+__device__ inline int64_t rb_at(const Rb& R, int a, int b, int c) {
+  return a * R.sx + b * R.sy + (c >> 1) + (((a + b + c) & 1) ? R.cs : 0);
+}
+
+// A point's slot i in its colour's array, that array (own) and the other
+// colour's (oth), where all six neighbours are: oth[i -+ sx], oth[i -+ sy],
+// and oth[i + xm], oth[i + xp] for c - 1 and c + 1.
+struct RbPt {
+  int64_t i;
+  int64_t own, oth;  // offsets of the two colours' arrays
+  int xm, xp;
+};
+
+__device__ inline RbPt rb_pt(const Rb& R, int a, int b, int c) {
+  const bool k = ((a + b + c) & 1) != 0, codd = (c & 1) != 0;
+  return RbPt{a * R.sx + b * R.sy + (c >> 1), k ? R.cs : 0, k ? 0 : R.cs, codd ? 0 : -1, codd ? 1 : 0};
+}
+
+// A e at the point (6 e - the neighbours, summed in the one-array order:
+// -x, +x, -y, +y, -z, +z in the level's a, b, c axes)
+template <class T>
+__device__ inline T rb_ae(const Rb& R, const RbPt& p) {
+  const float* eo = R.e + p.own;
+  const float* en = R.e + p.oth;
+  return (T)6 * (T)eo[p.i] - ((T)en[p.i - R.sx] + (T)en[p.i + R.sx] + (T)en[p.i - R.sy] + (T)en[p.i + R.sy] +
+                              (T)en[p.i + p.xm] + (T)en[p.i + p.xp]);
+}
+
+// Red-black half-sweep of colour `color` (k_smooth): thread j takes the j-th
+// point of the colour in each of its rows (c = 2j + 1 or 2j + 2).
+__global__ void k_smooth_rb(Rb R, int color, float w1, float w6, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int j = bx * blockDim.x + threadIdx.x;
+  const int a = bz + 1;
+  float* own = R.e + (color ? R.cs : 0);
+  const float* oth = R.e + (color ? 0 : R.cs);
+  const float* ro = R.r + (color ? R.cs : 0);
+  float val[kRows];
+  int64_t ii[kRows];
+  bool ok[kRows];
+#pragma unroll
+  for (int q = 0; q < kRows; ++q) {
+    const int b = by * kRows + q + 1;
+    const int odd = (a + b + color) & 1;  // the row's points of this colour have odd c
+    const int c = 2 * j + (odd ? 1 : 2), h = c >> 1;
+    ok[q] = b <= R.n1 && c <= R.n2;
+    // (a slot outside the level reads slot 1 of row 1 instead)
+    const int64_t i = ok[q] ? a * R.sx + b * R.sy + h : a * R.sx + R.sy + 1;
+    const bool codd = ok[q] && (c & 1) != 0;
+    ii[q] = i;
+    const float nb = oth[i - R.sx] + oth[i + R.sx] + oth[i - R.sy] + oth[i + R.sy] + oth[i + (codd ? 0 : -1)] +
+                     oth[i + (codd ? 1 : 0)];
+    val[q] = w1 * own[i] + w6 * (nb + ro[i]);
+  }
+#pragma unroll
+  for (int q = 0; q < kRows; ++q)
+    if (ok[q]) own[ii[q]] = val[q];
+}
+
+// First sweep of a cycle from e = 0 (k_smooth_first2): slot h of a row holds
+// one red and one black point.
+__global__ void k_smooth_first2_rb(Rb R, float w1, float w6, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int h = bx * blockDim.x + threadIdx.x;
+  const int a = bz + 1;
+  const float* r0 = R.r;
+  const float* r1 = R.r + R.cs;
+  float vr[kRows], vb[kRows];
+  int64_t ir[kRows], ib[kRows];
+  bool okr[kRows], okb[kRows];
+#pragma unroll
+  for (int q = 0; q < kRows; ++q) {
+    const int b = by * kRows + q + 1;
+    const int cr = 2 * h + ((a + b) & 1), cb = 2 * h + ((a + b + 1) & 1);
+    okr[q] = b <= R.n1 && cr >= 1 && cr <= R.n2;
+    okb[q] = b <= R.n1 && cb >= 1 && cb <= R.n2;
+    ir[q] = okr[q] ? a * R.sx + b * R.sy + h : a * R.sx + R.sy + 1;
+    ib[q] = okb[q] ? a * R.sx + b * R.sy + h : a * R.sx + R.sy + 1;
+    vr[q] = w6 * r0[ir[q]];
+    const int64_t i = ib[q];
+    const bool codd = okb[q] && (cb & 1) != 0;
+    const float nb = w6 * r0[i - R.sx] + w6 * r0[i + R.sx] + w6 * r0[i - R.sy] + w6 * r0[i + R.sy] +
+                     w6 * r0[i + (codd ? 0 : -1)] + w6 * r0[i + (codd ? 1 : 0)];
+    vb[q] = w1 * 0.0f + w6 * (nb + r1[i]);
+  }
+#pragma unroll
+  for (int q = 0; q < kRows; ++q) {
+    if (okr[q]) R.e[ir[q]] = vr[q];
+    if (okb[q]) R.e[R.cs + ib[q]] = vb[q];
+  }
+}
+
+// t = r - A e at every interior point, t in the one-array layout (k_resid_level).
+// (one thread per row slot h, both colours of the slot: every load of a
+// wavefront reads one contiguous run of a colour's array)
+__global__ void k_resid_level_rb(float* __restrict__ t, Rb R, Lv L, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int h = bx * blockDim.x + threadIdx.x;
+  const int a = bz + 1;
+#pragma unroll
+  for (int q = 0; q < kRows; ++q) {
+    const int b = by * kRows + q + 1;
+    if (b > L.n1) break;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int c = 2 * h + ((a + b + k) & 1);
+      if (c < 1 || c > L.n2) continue;
+      const RbPt pt = rb_pt(R, a, b, c);
+      t[a * L.sx + b * L.sy + c] = R.r[pt.own + pt.i] - rb_ae<float>(R, pt);
+    }
+  }
+}
+
+// e += trilinear interpolation of the coarse correction (k_prolong).
+__global__ void k_prolong_rb(Rb R, const float* __restrict__ ec, Lv C, const int* __restrict__ done) {
+  if (*done) return;
+  int bx, by, bz;
+  block3(bx, by, bz);
+  const int c = bx * blockDim.x + threadIdx.x + 1;
+  const int a = bz + 1;
+  if (c > R.n2) return;
+  const int a0 = a >> 1, a1 = (a + 1) >> 1, c0 = c >> 1, c1 = (c + 1) >> 1;
+  auto E = [&](int x, int y, int z) { return ec[x * C.sx + y * C.sy + z]; };
+#pragma unroll
+  for (int q = 0; q < kRows; ++q) {
+    const int b = by * kRows + q + 1;
+    if (b > R.n1) break;
+    const int b0 = b >> 1, b1 = (b + 1) >> 1;
+    const float v = (E(a0, b0, c0) + E(a1, b0, c0) + E(a0, b1, c0) + E(a1, b1, c0) + E(a0, b0, c1) + E(a1, b0, c1) +
+                     E(a0, b1, c1) + E(a1, b1, c1)) *
+                    0.125f;
+    R.e[rb_at(R, a, b, c)] += v;
+  }
+}
+
+// The colour-split analogue of for_interior_rows: each workgroup takes one
+// contiguous range of rows and walks their slots h (both colours of a slot
+// in one thread); f(g, a, b, c) gets every interior point once.
+template <class F>
+__device__ inline void for_interior_slots(const Rb& R, const int64_t sx, const int64_t sy, F&& f) {
+  const int nrows = R.n0 * R.n1, nh = (R.n2 >> 1) + 1;
+  const int per = (nrows + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int r0 = xcd_block(blockIdx.x, gridDim.x) * per;
+  const int r1 = min(r0 + per, nrows);
+  const int tpr = nh <= 64 ? 64 : (nh <= 128 ? 128 : 256);  // threads per row
+  const int sub = threadIdx.x / tpr, lane = threadIdx.x % tpr;
+  for (int row = r0 + sub; row < r1; row += 256 / tpr) {
+    const int a = row / R.n1 + 1, b = row % R.n1 + 1;
+    const int64_t base = a * sx + b * sy;
+    for (int h = lane; h < nh; h += tpr) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int c = 2 * h + ((a + b + k) & 1);
+        if (c >= 1 && c <= R.n2) f(base + c, a, b, c);
+      }
+    }
+  }
+}
+
+// The level-0 passes of the defect correction with e and r stored by colour
+// (k_resid0, k_update, k_pair_resid); phi, rho and the FP64 residual keep the
+// glue layout. The two that read and write FP64 arrays walk the points in
+// row order (a slot walk would touch those arrays with a stride of two);
+// the residual pass between the cycles of a pair walks the slots.
+__global__ __launch_bounds__(256) void k_resid0_rb(const double* __restrict__ phi, const double* __restrict__ rho,
+                                                   double rc, Rb R, bool store, Lv L, double* part,
+                                                   const int* __restrict__ done, double* __restrict__ r64) {
+  if (*done) return;
+  __shared__ double s1[256], s2[256];
+  double rr = 0.0, bb = 0.0;
+  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g, int a, int b, int c) {
+    const double bv = rc * rho[g];
+    const double ap = 6.0 * phi[g] - (phi[g - L.sx] + phi[g + L.sx] + phi[g - L.sy] + phi[g + L.sy] + phi[g - 1] +
+                                      phi[g + 1]);
+    const double rv = bv - ap;
+    if (store) {
+      const RbPt pt = rb_pt(R, a, b, c);
+      R.r[pt.own + pt.i] = (float)rv;
+      r64[g] = rv;
+    }
+    rr += rv * rv;
+    bb += bv * bv;
+  });
+  s1[threadIdx.x] = rr;
+  s2[threadIdx.x] = bb;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_update_rb(double* __restrict__ phi, Rb R, double* r64, Lv L, double* part,
+                                                   const int* __restrict__ done) {
+  if (*done) return;
+  __shared__ double s1[256];
+  double rr = 0.0;
+  const float* e = R.e;
+  for_interior_rows(L.n0, L.n1, L.n2, L.sx, L.sy, [&](const int64_t g, int a, int b, int c) {
+    const RbPt pt = rb_pt(R, a, b, c);
+    const double eg = (double)e[pt.own + pt.i];
+    const double ae = rb_ae<double>(R, pt);
+    const double rv = r64[g] - ae;
+    r64[g] = rv;
+    R.r[pt.own + pt.i] = (float)rv;
+    phi[g] += eg;
+    rr += rv * rv;
+  });
+  s1[threadIdx.x] = rr;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s1[threadIdx.x] += s1[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pair_resid_rb(Rb R, Lv L, double* part, const int* __restrict__ done) {
+  if (*done) return;
+  __shared__ double s1[256];
+  double rr = 0.0;
+  const float* e = R.e;
+  for_interior_slots(R, L.sx, L.sy, [&](const int64_t, int a, int b, int c) {
+    const RbPt pt = rb_pt(R, a, b, c);
+    const float t = R.r[pt.own + pt.i] - rb_ae<float>(R, pt);
+    rr += (double)t * (double)t;
+  });
+  s1[threadIdx.x] = rr;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s1[threadIdx.x] += s1[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = 0.0;
   }
 }
 
@@ -858,9 +1134,35 @@ TailArgs<P> tail_args(Ctx& c, const int* done) {
   return A;
 }
 
+Rb rb_of(Ctx& c) {
+  const MgLevel& m = c.mg[0];
+  return Rb{c.erb.p, c.rrb.p, c.rb_cs, c.rb_sx, c.rb_sy, m.n[0], m.n[1], m.n[2]};
+}
+
+// the colour-split finest level, kRows rows per thread: the row slots h that
+// hold interior points (0 .. n2 / 2) for the first sweep, which takes both
+// colours of a slot; a half-sweep takes one colour's points (grid_half4)
+dim3 grid_rb(const Ctx& c) {
+  const MgLevel& m = c.mg[0];
+  return dim3(div_up((m.n[2] >> 1) + 1, 64), div_up(m.n[1], kRows), m.n[0]);
+}
+
 template <class P>
 void smooth(Ctx& c, MgLevel& m, int sweeps, double w, bool from_zero, const int* done) {
   const P w1 = (P)(1.0 - w), w6 = (P)(w / 6.0);
+  if constexpr (std::is_same<P, float>::value) {
+    if (c.mg_rb && &m == &c.mg[0]) {
+      const Rb R = rb_of(c);
+      for (int s = 0; s < sweeps; ++s) {
+        if (from_zero && s == 0) {
+          k_smooth_first2_rb<<<grid_rb(c), 64, 0, c.stream>>>(R, w1, w6, done);
+          continue;
+        }
+        for (int col = 0; col < 2; ++col) k_smooth_rb<<<grid_half4(m), 64, 0, c.stream>>>(R, col, w1, w6, done);
+      }
+      return;
+    }
+  }
   for (int s = 0; s < sweeps; ++s) {
     if (from_zero && s == 0) {
       k_smooth_first2<P><<<grid_half4(m), 64, 0, c.stream>>>(E_<P>(m), R_<P>(m), lv_of(m), w1, w6, done);
@@ -900,7 +1202,14 @@ void vcycle(Ctx& c, int l, const int* done, bool zero = true) {
   // every level starts its cycle from e = 0 (the correction equation)
   if (zero && c.d.mg_pre == 0) k_zero_level<P><<<grid_full(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), done);
   smooth<P>(c, m, c.d.mg_pre, w, zero, done);
-  if ((int64_t)m.n[0] * m.n[1] * m.n[2] >= kSplitRestrict) {
+  bool rb = false;
+  if constexpr (std::is_same<P, float>::value) rb = c.mg_rb && l == 0;
+  if (rb) {
+    if constexpr (std::is_same<P, float>::value) {
+      k_resid_level_rb<<<grid_rb(c), 64, 0, c.stream>>>(T_<P>(m), rb_of(c), lv_of(m), done);
+      k_restrict_t<P><<<grid_full(mc), 64, 0, c.stream>>>(R_<P>(mc), lv_of(mc), T_<P>(m), lv_of(m), done);
+    }
+  } else if ((int64_t)m.n[0] * m.n[1] * m.n[2] >= kSplitRestrict) {
     k_resid_level<P><<<grid_full4(m), 64, 0, c.stream>>>(T_<P>(m), E_<P>(m), R_<P>(m), lv_of(m), done);
     k_restrict_t<P><<<grid_full(mc), 64, 0, c.stream>>>(R_<P>(mc), lv_of(mc), T_<P>(m), lv_of(m), done);
   } else {
@@ -908,7 +1217,12 @@ void vcycle(Ctx& c, int l, const int* done, bool zero = true) {
                                                             done);
   }
   vcycle<P>(c, l + 1, done);
-  k_prolong<P><<<grid_full4(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), E_<P>(mc), lv_of(mc), done);
+  if (rb) {
+    if constexpr (std::is_same<P, float>::value)
+      k_prolong_rb<<<grid_full4(m), 64, 0, c.stream>>>(rb_of(c), E_<P>(mc), lv_of(mc), done);
+  } else {
+    k_prolong<P><<<grid_full4(m), 64, 0, c.stream>>>(E_<P>(m), lv_of(m), E_<P>(mc), lv_of(mc), done);
+  }
   smooth<P>(c, m, c.d.mg_post, w, false, done);
 }
 
@@ -942,7 +1256,20 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   double* r64 = std::is_same<P, double>::value ? reinterpret_cast<double*>(r0) : c.mg_r64.p + off;
   const int* never = st + 3;  // stays 0: the final residual always runs
   k_solve_begin<<<1, 1, 0, c.stream>>>(st);
-  k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, part, done, r64);
+  // the level-0 passes, on the colour-split arrays when the level has them
+  bool rb = false;
+  if constexpr (std::is_same<P, float>::value) rb = c.mg_rb;
+  auto resid0 = [&](bool store, const int* flag) {
+    if constexpr (std::is_same<P, float>::value) {
+      if (rb) {
+        k_resid0_rb<<<nb, 256, 0, c.stream>>>(phi, rho, rc, rb_of(c), store, L0, part, flag, store ? r64 : nullptr);
+        return;
+      }
+    }
+    k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, store ? r0 : (P*)nullptr, L0, part, flag,
+                                          store ? r64 : (double*)nullptr);
+  };
+  resid0(true, done);
   k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, 0, 0);
   k_keep_start<<<1, 1, 0, c.stream>>>(dst);
   // Queue as many cycles as the previous two solves needed, then more. With
@@ -972,11 +1299,25 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
   auto cycle = [&](int nv) {
     vcycle<P>(c, 0, done);
     if (nv == 2) {
-      k_pair_resid<P><<<nb, 256, 0, c.stream>>>(E_<P>(m0), r0, L0, part, done);
+      if constexpr (std::is_same<P, float>::value) {
+        if (rb)
+          k_pair_resid_rb<<<nb, 256, 0, c.stream>>>(rb_of(c), L0, part, done);
+        else
+          k_pair_resid<P><<<nb, 256, 0, c.stream>>>(E_<P>(m0), r0, L0, part, done);
+      } else {
+        k_pair_resid<P><<<nb, 256, 0, c.stream>>>(E_<P>(m0), r0, L0, part, done);
+      }
       k_pair_check<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, skip);
       vcycle<P>(c, 0, skip, false);
     }
-    k_update<P><<<nb, 256, 0, c.stream>>>(phi, E_<P>(m0), r64, r0, L0, part, done);
+    if constexpr (std::is_same<P, float>::value) {
+      if (rb)
+        k_update_rb<<<nb, 256, 0, c.stream>>>(phi, rb_of(c), r64, L0, part, done);
+      else
+        k_update<P><<<nb, 256, 0, c.stream>>>(phi, E_<P>(m0), r64, r0, L0, part, done);
+    } else {
+      k_update<P><<<nb, 256, 0, c.stream>>>(phi, E_<P>(m0), r64, r0, L0, part, done);
+    }
     k_norm<<<1, 256, 0, c.stream>>>(part, nb, dst, st, tol, max_cycles, nv, 1, nv == 2 ? skip : nullptr);
   };
   if (c.mg_use_graph && (!c.mg_graph[0] || c.mg_graph_esz != sizeof(P) || c.mg_graph_tol != tol ||
@@ -1023,7 +1364,7 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
     FOPIS_HIP(hipStreamSynchronize(c.stream));
     if (!hs[0]) continue;
     // confirm on the true residual b - A phi before stopping
-    k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, (P*)nullptr, L0, part, never, (double*)nullptr);
+    resid0(false, never);
     k_norm_final<<<1, 256, 0, c.stream>>>(part, nb, dst);
     FOPIS_HIP(hipMemcpyAsync(hd, dst, sizeof(hd), hipMemcpyDeviceToHost, c.stream));
     FOPIS_HIP(hipStreamSynchronize(c.stream));
@@ -1031,7 +1372,7 @@ int solve_impl(Ctx& c, double tol, int max_cycles, double* relres_out, const dou
     if (hd[2] <= tol || zero_rhs || hs[1] >= max_cycles) break;
     // the recursively updated residual has drifted from the true one: the
     // further cycles start from the true residual
-    k_resid0<P><<<nb, 256, 0, c.stream>>>(phi, rho, rc, r0, L0, part, never, r64);
+    resid0(true, never);
     k_reopen<<<1, 1, 0, c.stream>>>(st);
   }
   const int cycles = hs[1];
@@ -1108,6 +1449,22 @@ void poisson_init(Ctx& c) {
   if (f32) {
     c.mg_r64.alloc(c.nnode);
     FOPIS_HIP(hipMemsetAsync(c.mg_r64.p, 0, c.nnode * sizeof(double), c.stream));
+  }
+  // the finest level stored by colour: FP32 levels, a level large enough to
+  // form its residual in a pass of its own (and so never inside the LDS tail),
+  // pre-smoothing (its first sweep starts from zero); FOPIS_MG_RB=0 keeps one array
+  const char* rbs = std::getenv("FOPIS_MG_RB");
+  const MgLevel& l0 = c.mg[0];
+  c.mg_rb = f32 && !(rbs && rbs[0] == '0') && c.mg.size() > 1 && c.d.mg_pre > 0 &&
+            (int64_t)l0.n[0] * l0.n[1] * l0.n[2] >= kSplitRestrict;
+  if (c.mg_rb) {
+    c.rb_sy = (l0.n[2] + 3) / 2;
+    c.rb_sx = (int64_t)(l0.n[1] + 2) * c.rb_sy;
+    c.rb_cs = (int64_t)(l0.n[0] + 2) * c.rb_sx;
+    c.erb.alloc(2 * c.rb_cs);
+    c.rrb.alloc(2 * c.rb_cs);
+    FOPIS_HIP(hipMemsetAsync(c.erb.p, 0, 2 * c.rb_cs * sizeof(float), c.stream));
+    FOPIS_HIP(hipMemsetAsync(c.rrb.p, 0, 2 * c.rb_cs * sizeof(float), c.stream));
   }
   c.mg_state.alloc(8);
   FOPIS_HIP(hipMemsetAsync(c.mg_state.p, 0, 8 * sizeof(int32_t), c.stream));

@@ -486,3 +486,38 @@ def test_poisson_graph_replays_match_eager_launches(monkeypatch, mg_precision):
         np.testing.assert_array_equal(a[3], b[3])
     for ctx in ctxs.values():
         ctx.close()
+
+
+@pytest.mark.parametrize("pairs", ["1", "0"])
+def test_colour_split_finest_level_matches_one_array(monkeypatch, pairs):
+    """An FP32 finest level of 127^3 unknowns (2^20 or more: it forms its
+    residual in a pass of its own) is stored by colour, red and black halves
+    apart; every kernel on it forms the same sums in the same order as on the
+    one-array layout (FOPIS_MG_RB=0), so successive warm-started solves give
+    the same potential bit for bit and the same cycles, with single cycles and
+    with pairs; the residual norms agree to rounding (the split passes add up
+    a row's points in another order)."""
+    n = 129
+    monkeypatch.setenv("FOPIS_MG_PAIRS", pairs)
+    ctxs = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("FOPIS_MG_RB", mode)
+        ctxs[mode] = F.Context(F.make_desc(n, n, n, [(-1.0, 1.0, 1.0)], eps0=1.0, precision=F.FP32,
+                                           mg_precision=F.FP32))
+    rng = np.random.default_rng(11)
+    shape = ctxs["1"].field_shape(F.RHO)
+    z, y, x = np.meshgrid(*[np.linspace(0, 1, s) for s in shape], indexing="ij")
+    base = np.sin(3 * np.pi * x) * np.sin(2 * np.pi * y) * np.sin(np.pi * z)
+    out = {m: [] for m in ctxs}
+    for _ in range(3):
+        rho = base * rng.uniform(0.5, 2.0) + rng.normal(size=shape) * 0.2
+        for m, ctx in ctxs.items():
+            ctx.set_field(F.RHO, rho, 0)
+            cyc, rr = ctx.solve_poisson()  # warm-started from the previous potential
+            out[m].append((cyc, rr, ctx.get_field(F.PHI)))
+    for a, b in zip(out["1"], out["0"]):
+        assert a[0] == b[0] and a[0] > 0 and a[1] <= 1e-8 and b[1] <= 1e-8, (a[:2], b[:2])
+        assert abs(a[1] - b[1]) <= 1e-12 * b[1], (a[:2], b[:2])
+        np.testing.assert_array_equal(a[2], b[2])
+    for ctx in ctxs.values():
+        ctx.close()
