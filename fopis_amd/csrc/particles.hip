@@ -813,6 +813,11 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     return valid;
   };
 
+  // Particles absorbed at a domain face: one count per wavefront, added to the
+  // launch's total once at the tile's end. (An atomic per iteration stalled
+  // the face tiles: all of them run at once at a launch's start, and their
+  // wavefronts queued on the one counter.)
+  int nlost = 0;
   for (int c0 = 0; c0 < ntot; c0 += PPT * kBlock) {
     R p[PPT][6];
 #pragma unroll
@@ -829,11 +834,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
       valid[u] = work(p[u], c0 + u * kBlock + threadIdx.x, dt[u], lost[u]);
     }
     if constexpr ((MODE & M_POS) != 0) {
-      // particles absorbed at a domain face, one atomic per wavefront
-      int nl = 0;
+      // particles absorbed at a domain face, counted per wavefront
 #pragma unroll
-      for (int u = 0; u < PPT; ++u) nl += __popcll(__ballot(lost[u]));
-      if ((threadIdx.x & 63) == 0 && nl > 0) atomicAdd(&a.status[4], nl);
+      for (int u = 0; u < PPT; ++u) nlost += __popcll(__ballot(lost[u]));
     }
 
     if constexpr ((MODE & M_BIN) != 0) {
@@ -965,6 +968,9 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
     }
   }
 
+  if constexpr ((MODE & M_POS) != 0) {
+    if ((threadIdx.x & 63) == 0 && nlost > 0) atomicAdd(&a.status[4], nlost);
+  }
   if constexpr (DB) __syncthreads();  // every wavefront's reservations are in
   if constexpr ((MODE & M_BIN) != 0) {
     if (threadIdx.x == 0) {
