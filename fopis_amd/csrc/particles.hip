@@ -994,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_tile(TileArgs<R> a) {
 // its outbox to the tile in the entry's direction, after that tile's
 // stayers. A wavefront reserves the slots of all its entries bound for one
 // tile with one atomic on that tile's arrival count (one per distinct
-// direction among its 64 entries), so each outbox is read once. Arrivals
+// direction among its 64 entries, issued together), so each outbox is read once. Arrivals
 // past a tile's capacity go to the stray list. k_arrive_done then adds the
 // arrivals to the tiles' counts and clears them for the next step.
 template <class R>
@@ -1011,22 +1011,27 @@ __global__ __launch_bounds__(kBlock) void k_scatter(TileArgs<R> a, int32_t* __re
     const int d = act ? (int)a.odir[ob + i] : 13;
     // the destination tile (the binning pass only files directions of existing neighbours)
     const int t = ((sx + d / 9 - 1) * a.T[1] + (sy + (d / 3) % 3 - 1)) * a.T[2] + (sz + d % 3 - 1);
-    unsigned long long todo = __ballot(act);
-    int pos = 0;
-    while (todo) {
-      const int leader = __ffsll((long long)todo) - 1;
-      const int dl = __shfl(d, leader, 64);
-      const unsigned long long grp = __ballot(act && d == dl) & todo;
-      int b = 0;
-      if (lane == leader) b = atomicAdd(&arr[__shfl(t, leader, 64)], __popcll(grp));
-      b = __shfl(b, leader, 64);
-      if (act && d == dl) pos = b + __popcll(grp & lt);
-      todo &= ~grp;
-    }
-    if (act) {
-      R p[6];
+    // the entry, loaded before the reservation so that its loads are in flight meanwhile
+    R p[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) p[c] = a.out.c[c][ob + i];
+    for (int c = 0; c < 6; ++c) p[c] = act ? a.out.c[c][ob + i] : (R)0;
+    const int ct = a.cnt[t], cp = a.capa[t];
+    // grp: the lanes filing into the same direction as this lane (ballots
+    // only, one per distinct direction); then the lowest lane of each group
+    // reserves the group's slots, all groups' atomics in flight together
+    unsigned long long grp = 0;
+    unsigned long long todo = __ballot(act);
+    while (todo) {
+      const int dl = __builtin_amdgcn_readlane(d, __ffsll((long long)todo) - 1);
+      const unsigned long long g = __ballot(act && d == dl) & todo;
+      if (act && d == dl) grp = g;
+      todo &= ~g;
+    }
+    int b = 0;
+    if (act && (grp & lt) == 0) b = atomicAdd(&arr[t], __popcll(grp));
+    b = __shfl(b, act ? __ffsll((long long)grp) - 1 : lane, 64);
+    const int pos = b + __popcll(grp & lt);
+    if (act) {
       if constexpr (sizeof(R) == 4) {
         // FP32 outbox entries are relative to the source tile's origin; the
         // destination's origin lies one tile width away per step of d
@@ -1034,8 +1039,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(TileArgs<R> a, int32_t* __re
         p[1] += (R)(kTile * (1 - (d / 3) % 3));
         p[2] += (R)(kTile * (1 - d % 3));
       }
-      const int at = a.cnt[t] + pos;
-      if (at < a.capa[t]) {
+      const int at = ct + pos;
+      if (at < cp) {
         const int64_t g = a.start[t] + at;
 #pragma unroll
         for (int c = 0; c < 6; ++c) a.dst.c[c][g] = p[c];
