@@ -107,8 +107,6 @@ struct Cond {
   const uint8_t* mask;
   const int32_t* node_cmat;
   double* tmp;
-  unsigned long long* absorbed;
-  unsigned long long* lost;
   __device__ bool valid_cell(int i, int j, int k) const {
     return i >= 0 && i < cx - 1 && j >= 0 && j < cy - 1 && k >= 0 && k < cz - 1;
   }
@@ -176,8 +174,9 @@ __device__ __forceinline__ double move_ratio(double d1, double d2, double v) {
 
 // Spacecraft::distributeInnerParticleCharge with Particle::computeCrossPoints
 // (particle.cpp:82-204): the charge goes to the four cmat nodes of the first
-// crossed face whose orientation matches the velocity sign.
-__device__ void absorb(const Cond& cd, double x, double y, double z, double vx, double vy, double vz) {
+// crossed face whose orientation matches the velocity sign. Returns whether
+// such a face was found.
+__device__ bool absorb(const Cond& cd, double x, double y, double z, double vx, double vy, double vz) {
   DPos pos, old;
   set_xyz(pos, x, y, z);
   set_xyz(old, x - vx, y - vy, z - vz);
@@ -259,22 +258,39 @@ __device__ void absorb(const Cond& cd, double x, double y, double z, double vx, 
       found = true;
     }
   }
-  // the reference reports "Surface cannot detect" and still counts the
-  // particle in the current (Spacecraft.cpp:573-589); counted here instead
-  if (!found) atomicAdd(cd.lost, 1ull);
-  atomicAdd(cd.absorbed, 1ull);
+  return found;
 }
 
 // Surface charge of the particles captured in this step (one thread each).
+// The reference counts every captured particle in its object's current and
+// reports those whose face it cannot find ("Surface cannot detect",
+// Spacecraft.cpp:573-589); both counts are added once per wavefront and
+// object (an atomic per particle queued on the one counter).
 __global__ void k_capture(const double* rec, int n, int cx, int cy, int cz, double q, const uint8_t* mask,
                           ObjArgs o0, ObjArgs o1, ObjArgs o2, ObjArgs o3) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double* r = rec + (size_t)8 * i;
-  const int o = (int)r[6];
-  const ObjArgs oa = o == 0 ? o0 : (o == 1 ? o1 : (o == 2 ? o2 : o3));
-  Cond cd{cx, cy, cz, o, q, mask, oa.node_cmat, oa.tmp, oa.absorbed, oa.lost};
-  absorb(cd, r[0], r[1], r[2], r[3], r[4], r[5]);
+  const bool act = i < n;
+  int o = -1;
+  bool found = false;
+  if (act) {
+    const double* r = rec + (size_t)8 * i;
+    o = (int)r[6];
+    const ObjArgs oa = o == 0 ? o0 : (o == 1 ? o1 : (o == 2 ? o2 : o3));
+    Cond cd{cx, cy, cz, o, q, mask, oa.node_cmat, oa.tmp};
+    found = absorb(cd, r[0], r[1], r[2], r[3], r[4], r[5]);
+  }
+  unsigned long long todo = __ballot(act);
+  while (todo) {
+    const int ol = __builtin_amdgcn_readlane(o, __ffsll((long long)todo) - 1);
+    const unsigned long long g = __ballot(act && o == ol) & todo;
+    const unsigned long long gl = __ballot(act && o == ol && !found) & todo;
+    if ((threadIdx.x & 63) == __ffsll((long long)g) - 1) {
+      const ObjArgs oa = ol == 0 ? o0 : (ol == 1 ? o1 : (ol == 2 ? o2 : o3));
+      atomicAdd(oa.absorbed, (unsigned long long)__popcll(g));
+      if (gl) atomicAdd(oa.lost, (unsigned long long)__popcll(gl));
+    }
+    todo &= ~g;
+  }
 }
 
 constexpr int kWaves = kBlock / 64;  // wavefronts per block
