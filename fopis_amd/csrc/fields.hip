@@ -17,11 +17,12 @@ namespace {
 __device__ __forceinline__ int fdivd(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 __device__ __forceinline__ int cdivd(int a, int b) { return -fdivd(-a, b); }
 
-// One thread per glue node: rho[s+1] += sum of the windows of every tile
-// covering the node, in tile order (deterministic); rho[0] = sum over s.
+// One thread per glue node: rho[s+1] = the far deposits (only for species in
+// far_mask; read and cleared) plus the windows of every tile covering the
+// node, in tile order (deterministic); rho[0] = sum over s.
 template <class R>
-__global__ void k_reduce_rho(const R* __restrict__ part, double* __restrict__ rho, int S, int ntiles, int cx, int cy,
-                             int cz, int T0, int T1, int T2) {
+__global__ void k_reduce_rho(const R* __restrict__ part, double* __restrict__ rho, double* __restrict__ far,
+                             uint32_t far_mask, int S, int ntiles, int cx, int cy, int cz, int T0, int T1, int T2) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nn = (int64_t)cx * cy * cz;
   if (g >= nn) return;
@@ -33,7 +34,11 @@ __global__ void k_reduce_rho(const R* __restrict__ part, double* __restrict__ rh
   double tot = 0.0;
   for (int s = 0; s < S; ++s) {
     const R* P = part + (size_t)s * ntiles * (kRW * kRW * kRW);
-    double acc = rho[(size_t)(s + 1) * nn + g];
+    double acc = 0.0;
+    if ((far_mask >> s) & 1u) {
+      acc = far[(size_t)s * nn + g];
+      far[(size_t)s * nn + g] = 0.0;
+    }
     for (int tx = a0; tx <= b0; ++tx)
       for (int ty = a1; ty <= b1; ++ty)
         for (int tz = a2; tz <= b2; ++tz) {
@@ -47,9 +52,11 @@ __global__ void k_reduce_rho(const R* __restrict__ part, double* __restrict__ rh
   rho[g] = tot;
 }
 
-// One thread per density cell ((cx-1)^3, glue-relative).
-__global__ void k_reduce_dens(const int32_t* __restrict__ part, double* __restrict__ dens, const double* size, int S,
-                              int ntiles, int cx, int cy, int cz, int T0, int T1, int T2) {
+// One thread per density cell ((cx-1)^3, glue-relative): density[s] = the far
+// counts (species in far_mask; read and cleared) plus the windows' counts.
+__global__ void k_reduce_dens(const int32_t* __restrict__ part, double* __restrict__ dens, double* __restrict__ far,
+                              uint32_t far_mask, const double* size, int S, int ntiles, int cx, int cy, int cz, int T0,
+                              int T1, int T2) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int dx_ = cx - 1, dy_ = cy - 1, dz_ = cz - 1;
   const int64_t nn = (int64_t)dx_ * dy_ * dz_;
@@ -71,7 +78,12 @@ __global__ void k_reduce_dens(const int32_t* __restrict__ part, double* __restri
         }
     // density += size once per particle (RootGridField.cpp:1142): n additions
     // of an integer-valued size are exact, so n * size equals them.
-    dens[(size_t)s * nn + g] += (double)n * size[s];
+    double base = 0.0;
+    if ((far_mask >> s) & 1u) {
+      base = far[(size_t)s * nn + g];
+      far[(size_t)s * nn + g] = 0.0;
+    }
+    dens[(size_t)s * nn + g] = base + (double)n * size[s];
   }
 }
 
@@ -199,18 +211,18 @@ void fields_init(Ctx& c) {
   FOPIS_HIP(hipStreamSynchronize(c.stream));
 }
 
-void fields_reduce_windows(Ctx& c, bool do_rho, bool do_dens) {
+void fields_reduce_windows(Ctx& c, bool do_rho, bool do_dens, uint32_t far_mask) {
   hipStream_t st = c.stream;
   if (do_rho) {
     const int64_t nn = (int64_t)c.nnode;
     // charge windows are FP64 for both particle precisions
-    k_reduce_rho<double><<<div_up(nn, 256), 256, 0, st>>>(c.rho_part.p, c.rho.p, c.S, c.ntiles, c.cx, c.cy, c.cz,
-                                                         c.nt[0], c.nt[1], c.nt[2]);
+    k_reduce_rho<double><<<div_up(nn, 256), 256, 0, st>>>(c.rho_part.p, c.rho.p, c.rho_far.p, far_mask, c.S,
+                                                         c.ntiles, c.cx, c.cy, c.cz, c.nt[0], c.nt[1], c.nt[2]);
   }
   if (do_dens) {
-    k_reduce_dens<<<div_up((int64_t)c.ncell_d, 256), 256, 0, st>>>(c.dens_part.p, c.density.p, sizes_buf(c).p, c.S,
-                                                                   c.ntiles, c.cx, c.cy, c.cz, c.nt[0], c.nt[1],
-                                                                   c.nt[2]);
+    k_reduce_dens<<<div_up((int64_t)c.ncell_d, 256), 256, 0, st>>>(c.dens_part.p, c.density.p, c.dens_far.p,
+                                                                   far_mask, sizes_buf(c).p, c.S, c.ntiles, c.cx,
+                                                                   c.cy, c.cz, c.nt[0], c.nt[1], c.nt[2]);
   }
   FOPIS_HIP(hipGetLastError());
 }

@@ -270,6 +270,11 @@ struct Ctx {
   // fields (FP64, reference glue layout)
   DBuf<double> phi, rho, density, eref;   // rho: (S+1)*nnode, density: S*ncell_d, eref: 3*nnode
   DBuf<double> rho_part;                  // [ntiles][kRW^3] per species launch
+  // per species, in the layouts of rho[s + 1] and density[s]: the deposits and
+  // counts the tile kernel adds with global atomics (a particle outside its
+  // tile's windows); all zero between steps (the window reduction that reads
+  // them clears them)
+  DBuf<double> rho_far, dens_far;
   DBuf<int32_t> dens_part;                // [ntiles][kDW^3]
   DBuf<double> red;                       // reduction scratch
   DBuf<int32_t> status;                   // [8] stray counts etc.
@@ -365,7 +370,8 @@ void particles_ledger(Ctx& c, int s, int64_t* out);
 
 void fields_init(Ctx& c);
 void fields_efield(Ctx& c);
-void fields_reduce_windows(Ctx& c, bool do_rho, bool do_dens);
+// far_mask: bit s set when species s may have added to rho_far / dens_far
+void fields_reduce_windows(Ctx& c, bool do_rho, bool do_dens, uint32_t far_mask = ~0u);
 void fields_get(Ctx& c, int which, int s, double* out);
 void fields_set(Ctx& c, int which, int s, const double* in);
 double fields_energy(Ctx& c);

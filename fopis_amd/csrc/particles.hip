@@ -1636,8 +1636,8 @@ TileArgs<R> tile_args(Ctx& c, int s) {
   a.status = c.status.p + kStat * s;
   a.rho_part = c.rho_part.p + (size_t)s * c.ntiles * (kRW * kRW * kRW);
   a.dens_part = c.dens_part.p + (size_t)s * c.ntiles * (kDW * kDW * kDW);
-  a.rho_g = c.rho.p + (size_t)(s + 1) * c.nnode;
-  a.dens_g = c.density.p + (size_t)s * c.ncell_d;
+  a.rho_g = c.rho_far.p + (size_t)s * c.nnode;
+  a.dens_g = c.dens_far.p + (size_t)s * c.ncell_d;
   a.mask = c.mask.p;
   a.tile_near = c.tile_near.p;
   a.sink = reinterpret_cast<R*>(c.sink.p);
@@ -1857,8 +1857,9 @@ void launch_tiles_db(Ctx& c, int s) {
   FOPIS_HIP(hipGetLastError());
 }
 
+// Returns the species that deposited or counted past their tile windows.
 template <class R>
-void after_bin(Ctx& c) {
+uint32_t after_bin(Ctx& c) {
   std::vector<int32_t> stat(kStat * c.S);
   FOPIS_HIP(hipMemcpyAsync(stat.data(), c.status.p, kStat * c.S * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
   FOPIS_HIP(hipStreamSynchronize(c.stream));
@@ -1895,6 +1896,10 @@ void after_bin(Ctx& c) {
   // strays -> full re-bin of that species
   for (int s = 0; s < c.S; ++s)
     if (stat[kStat * s + 0] > 0) rebin<R>(c, s);
+  uint32_t far = 0;
+  for (int s = 0; s < c.S; ++s)
+    if (stat[kStat * s + 2] > 0) far |= 1u << s;
+  return far;
 }
 
 template <class R>
@@ -1902,8 +1907,7 @@ void deposit_bin(Ctx& c, bool push) {
   hipStream_t st = c.stream;
   Phase whole(c, "particles");
   for (int s = 0; s < c.S; ++s) prepare_inbox<R>(c, s, true);
-  FOPIS_HIP(hipMemsetAsync(c.rho.p, 0, c.rho.n * sizeof(double), st));
-  FOPIS_HIP(hipMemsetAsync(c.density.p, 0, c.density.n * sizeof(double), st));
+  // (rho and density are written whole by the window reduction)
   FOPIS_HIP(hipMemsetAsync(c.status.p, 0, c.status.n * sizeof(int32_t), st));
   for (int o = 0; o < c.nobj; ++o)
     if (c.obj[o].absorbed.p)
@@ -1942,10 +1946,10 @@ void deposit_bin(Ctx& c, bool push) {
     bins<R>(c)[s].inbox_n = 0;
     bins<R>(c)[s].has_invalid = false;  // re-binning dropped every invalid particle
   }
-  after_bin<R>(c);
+  const uint32_t far_mask = after_bin<R>(c);
   {
     Phase ph(c, "reduce");
-    fields_reduce_windows(c, true, true);
+    fields_reduce_windows(c, true, true, far_mask);
   }
 }
 
@@ -1960,6 +1964,10 @@ void particles_init(Ctx& c) {
     c.sp64.resize(c.S);
   // rho_part holds S*ntiles FP64 charge windows (both particle precisions)
   c.rho_part.alloc((size_t)c.S * c.ntiles * (kRW * kRW * kRW));
+  c.rho_far.alloc((size_t)c.S * c.nnode);
+  c.dens_far.alloc((size_t)c.S * c.ncell_d);
+  FOPIS_HIP(hipMemsetAsync(c.rho_far.p, 0, c.rho_far.n * sizeof(double), c.stream));
+  FOPIS_HIP(hipMemsetAsync(c.dens_far.p, 0, c.dens_far.n * sizeof(double), c.stream));
   c.dens_part.alloc((size_t)c.S * c.ntiles * (kDW * kDW * kDW));
   c.cap_cap = 1 << 20;  // captured particles per species and step
   c.cap_rec.alloc((size_t)8 * c.cap_cap * c.S);
